@@ -1,0 +1,51 @@
+/*
+ * rawspec_amd.h -- extensions of librawspec beyond the reference API.
+ *
+ * None of these has a counterpart in the reference (rawspec.h): they serve
+ * GPU-resident producers and consumers that should not pay a PCIe round trip
+ * per buffer (a capture pipeline writing voltages straight into device
+ * memory, a GPU search consuming spectra in place), and the throughput
+ * measurement, which times the device-side path with inputs already in HBM.
+ * The reference API in rawspec.h keeps working unchanged alongside them.
+ */
+#ifndef RAWSPEC_AMD_H_
+#define RAWSPEC_AMD_H_
+
+#include <stddef.h>
+
+#include "rawspec.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device pointer and size of the context's input buffer, laid out
+ * [Nc][Nb*Ntpb][Np][re,im] with Nbps-bit components (the layout
+ * rawspec_copy_blocks_to_gpu builds).  NULL / 0 before initialisation. */
+void  *rawspec_amd_device_input(rawspec_context *ctx);
+size_t rawspec_amd_device_input_size(rawspec_context *ctx);
+
+/* As rawspec_start_processing(), reading the input from d_in, a device
+ * buffer of rawspec_amd_device_input_size() bytes in the same layout on the
+ * context's GPU.  d_in must stay unchanged until the work completes. */
+int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in,
+                                        int fft_dir);
+
+/* enable != 0: dumps stay in device memory (no copy into h_pwrbuf/h_icsbuf);
+ * the PRE/POST callbacks still fire in stream order.  The dumped spectra of
+ * product i (same layout as h_pwrbuf[i]) are then valid at
+ * rawspec_amd_device_pwrbuf(ctx, i) from POST(i) until the next call that
+ * starts processing on this context. */
+int rawspec_amd_set_device_output(rawspec_context *ctx, int enable);
+float *rawspec_amd_device_pwrbuf(rawspec_context *ctx, int i);
+float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i);
+
+/* The context's HIP stream (a hipStream_t), for callers that order their
+ * own device work after the context's. */
+void *rawspec_amd_stream(rawspec_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

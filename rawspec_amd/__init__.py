@@ -1,0 +1,260 @@
+"""Python mirror of librawspec's C API (include/rawspec.h, include/rawspec_amd.h).
+
+The product is the C-ABI shared library ``rawspec_amd/librawspec.so`` (C host
+layer + gfx950 HIP kernels).  This module binds it with ctypes so tests,
+benchmarks and Python clients drive exactly the entry points a C client
+would: ``RawspecContext`` is field-for-field ``struct rawspec_context_s``
+(reference rawspec.h:32-148) and every ``rawspec_*`` function keeps the
+reference's name, arguments, return codes and error behaviour.
+
+Loading fails loudly when the library has not been built; there is no
+fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "MAX_OUTPUTS", "RAWSPEC_FORWARD_FFT", "RAWSPEC_INVERSE_FFT",
+    "RAWSPEC_CALLBACK_PRE_DUMP", "RAWSPEC_CALLBACK_POST_DUMP",
+    "RawspecContext", "DumpCallback", "lib", "lib_path", "Spectrometer",
+]
+
+MAX_OUTPUTS = 4
+RAWSPEC_FORWARD_FFT = +1
+RAWSPEC_INVERSE_FFT = -1
+RAWSPEC_CALLBACK_PRE_DUMP = 0
+RAWSPEC_CALLBACK_POST_DUMP = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "librawspec.so")
+
+
+class RawspecContext(C.Structure):
+    pass
+
+
+# void (*)(rawspec_context *ctx, int output_product, int callback_type)
+DumpCallback = C.CFUNCTYPE(None, C.POINTER(RawspecContext), C.c_int, C.c_int)
+
+RawspecContext._fields_ = [
+    ("No", C.c_uint),
+    ("Np", C.c_uint),
+    ("Nant", C.c_uint),
+    ("Nc", C.c_uint),
+    ("Ntpb", C.c_uint),
+    ("exit_soon", C.c_uint),
+    ("Nbc", C.c_uint),
+    ("Nbps", C.c_uint),
+    ("Npolout", C.c_int * MAX_OUTPUTS),
+    ("Nts", C.c_uint * MAX_OUTPUTS),
+    ("Nas", C.c_uint * MAX_OUTPUTS),
+    ("dump_callback", DumpCallback),
+    ("user_data", C.c_void_p),
+    ("Nb", C.c_uint),
+    ("Nb_host", C.c_uint),
+    ("h_blkbufs", C.POINTER(C.c_char_p)),
+    ("gpu_index", C.c_int),
+    ("input_conjugated", C.c_int),
+    ("incoherently_sum", C.c_int),
+    ("Naws", C.c_int),
+    ("Aws", C.POINTER(C.c_float)),
+    ("h_pwrbuf", C.POINTER(C.c_float) * MAX_OUTPUTS),
+    ("h_pwrbuf_size", C.c_size_t * MAX_OUTPUTS),
+    ("h_icsbuf", C.POINTER(C.c_float) * MAX_OUTPUTS),
+    ("Nds", C.c_uint * MAX_OUTPUTS),
+    ("Ntmax", C.c_uint),
+    ("gpu_ctx", C.c_void_p),
+]
+
+_lib = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def lib():
+    """Loads librawspec.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(
+                f"{_LIB_PATH} is missing: build it with `python -m rawspec_amd.build` "
+                "(there is no fallback implementation)")
+        L = C.CDLL(_LIB_PATH)
+        P = C.POINTER(RawspecContext)
+        L.get_librawspec_version.restype = C.c_char_p
+        L.get_librawspec_version.argtypes = []
+        L.get_cufft_version.restype = C.c_char_p
+        L.get_cufft_version.argtypes = []
+        for name, args, res in [
+            ("rawspec_initialize", [P], C.c_int),
+            ("rawspec_cleanup", [P], None),
+            ("rawspec_copy_blocks_to_gpu", [P, C.c_long, C.c_long, C.c_size_t], C.c_int),
+            ("rawspec_copy_blocks_to_gpu_expanding_complex4",
+             [P, C.c_long, C.c_long, C.c_size_t], C.c_int),
+            ("rawspec_zero_blocks_to_gpu", [P, C.c_long, C.c_size_t], C.c_int),
+            ("rawspec_start_processing", [P, C.c_int], C.c_int),
+            ("rawspec_copy_blocks_to_gpu_and_start_processing",
+             [P, C.c_size_t, C.c_char, C.c_int], C.c_int),
+            ("rawspec_reset_integration", [P], C.c_int),
+            ("rawspec_check_for_completion", [P], C.c_uint),
+            ("rawspec_wait_for_completion", [P], C.c_int),
+            ("rawspec_amd_device_input", [P], C.c_void_p),
+            ("rawspec_amd_device_input_size", [P], C.c_size_t),
+            ("rawspec_amd_start_processing_device", [P, C.c_void_p, C.c_int], C.c_int),
+            ("rawspec_amd_set_device_output", [P, C.c_int], C.c_int),
+            ("rawspec_amd_device_pwrbuf", [P, C.c_int], C.c_void_p),
+            ("rawspec_amd_device_icsbuf", [P, C.c_int], C.c_void_p),
+            ("rawspec_amd_stream", [P], C.c_void_p),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+class Spectrometer:
+    """Thin convenience owner of one ``rawspec_context``.
+
+    Fields are set exactly as a C client would set them; methods call the
+    C entry points one to one.  Host block buffers, power buffers and the
+    ICS buffers are exposed as numpy views of the library's pinned memory.
+    """
+
+    def __init__(self, *, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None,
+                 Nant=1, Nbc=0, Nb=0, Nb_host=0, gpu_index=0,
+                 input_conjugated=0, incoherently_sum=0, Aws=None,
+                 callback=None, init=True):
+        self.ctx = RawspecContext()
+        c = self.ctx
+        No = len(Nts)
+        c.No, c.Np, c.Nant, c.Nc, c.Ntpb = No, Np, Nant, Nc, Ntpb
+        c.Nbc, c.Nbps, c.Nb, c.Nb_host = Nbc, Nbps, Nb, Nb_host
+        Npolout = list(Npolout) if Npolout is not None else [1] * No
+        for i in range(No):
+            c.Nts[i] = Nts[i]
+            c.Nas[i] = Nas[i]
+            c.Npolout[i] = Npolout[i]
+        c.gpu_index = gpu_index
+        c.input_conjugated = input_conjugated
+        c.incoherently_sum = incoherently_sum
+        self._aws = None
+        if Aws is not None:
+            self._aws = (C.c_float * len(Aws))(*Aws)
+            c.Aws = C.cast(self._aws, C.POINTER(C.c_float))
+            c.Naws = len(Aws)
+        self.events = []   # (product, type) in callback order
+        self._user_cb = callback
+        self._cb = DumpCallback(self._on_dump)
+        c.dump_callback = self._cb
+        self.initialized = False
+        if init:
+            rc = self.initialize()
+            if rc != 0:
+                raise RuntimeError(f"rawspec_initialize failed ({rc})")
+
+    def _on_dump(self, ctxp, product, cbtype):
+        self.events.append((product, cbtype))
+        if self._user_cb is not None:
+            self._user_cb(self, product, cbtype)
+
+    # -- C entry points --------------------------------------------------
+    def initialize(self) -> int:
+        rc = lib().rawspec_initialize(C.byref(self.ctx))
+        self.initialized = rc == 0
+        return rc
+
+    def cleanup(self):
+        if self.initialized:
+            lib().rawspec_cleanup(C.byref(self.ctx))
+            self.initialized = False
+
+    def copy_blocks(self, src_idx=0, dst_idx=0, num_blocks=None) -> int:
+        n = self.ctx.Nb if num_blocks is None else num_blocks
+        return lib().rawspec_copy_blocks_to_gpu(C.byref(self.ctx), src_idx, dst_idx, n)
+
+    def copy_blocks_expanding_complex4(self, src_idx=0, dst_idx=0, num_blocks=None) -> int:
+        n = self.ctx.Nb if num_blocks is None else num_blocks
+        return lib().rawspec_copy_blocks_to_gpu_expanding_complex4(
+            C.byref(self.ctx), src_idx, dst_idx, n)
+
+    def zero_blocks(self, dst_idx=0, num_blocks=1) -> int:
+        return lib().rawspec_zero_blocks_to_gpu(C.byref(self.ctx), dst_idx, num_blocks)
+
+    def start_processing(self, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
+        return lib().rawspec_start_processing(C.byref(self.ctx), fft_dir)
+
+    def copy_and_start(self, num_blocks=None, expand4=False, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
+        n = self.ctx.Nb if num_blocks is None else num_blocks
+        return lib().rawspec_copy_blocks_to_gpu_and_start_processing(
+            C.byref(self.ctx), n, b"\x01" if expand4 else b"\x00", fft_dir)
+
+    def start_processing_device(self, dev_ptr: int, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
+        return lib().rawspec_amd_start_processing_device(C.byref(self.ctx), C.c_void_p(dev_ptr),
+                                                         fft_dir)
+
+    def set_device_output(self, enable: bool) -> int:
+        return lib().rawspec_amd_set_device_output(C.byref(self.ctx), 1 if enable else 0)
+
+    def device_input(self) -> int:
+        return lib().rawspec_amd_device_input(C.byref(self.ctx)) or 0
+
+    def device_input_size(self) -> int:
+        return lib().rawspec_amd_device_input_size(C.byref(self.ctx))
+
+    def device_pwrbuf(self, i) -> int:
+        return lib().rawspec_amd_device_pwrbuf(C.byref(self.ctx), i) or 0
+
+    def stream(self) -> int:
+        return lib().rawspec_amd_stream(C.byref(self.ctx)) or 0
+
+    def reset_integration(self) -> int:
+        return lib().rawspec_reset_integration(C.byref(self.ctx))
+
+    def check_for_completion(self) -> int:
+        return lib().rawspec_check_for_completion(C.byref(self.ctx))
+
+    def wait_for_completion(self) -> int:
+        return lib().rawspec_wait_for_completion(C.byref(self.ctx))
+
+    # -- numpy views -----------------------------------------------------
+    @property
+    def blocsize(self) -> int:
+        c = self.ctx
+        return c.Nc * c.Ntpb * c.Np * 2 * c.Nbps // 8
+
+    def block(self, i) -> np.ndarray:
+        """Writable uint8 view of host block buffer i."""
+        ptr = C.cast(self.ctx.h_blkbufs[i], C.c_void_p).value
+        return np.ctypeslib.as_array((C.c_uint8 * self.blocsize).from_address(ptr))
+
+    def pwrbuf(self, i) -> np.ndarray:
+        n = self.ctx.h_pwrbuf_size[i] // 4
+        ptr = C.cast(self.ctx.h_pwrbuf[i], C.c_void_p).value
+        return np.ctypeslib.as_array((C.c_float * n).from_address(ptr))
+
+    def icsbuf(self, i) -> np.ndarray:
+        n = self.ctx.h_pwrbuf_size[i] // 4 // self.ctx.Nant
+        ptr = C.cast(self.ctx.h_icsbuf[i], C.c_void_p).value
+        return np.ctypeslib.as_array((C.c_float * n).from_address(ptr))
+
+    def close(self):
+        self.cleanup()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,102 @@
+"""Builds librawspec.so (C host layer + gfx950 HIP kernels) in-tree.
+
+    python -m rawspec_amd.build          # compile and link
+    python -m rawspec_amd.build --check  # also print the exported symbols
+
+The library lands at rawspec_amd/librawspec.so.  Objects go to
+rawspec_amd/_build/.  hipcc cross-compiles for gfx950 without a GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INC = os.path.join(ROOT, "include")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "librawspec.so")
+
+ARCH = os.environ.get("RAWSPEC_AMD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+
+HIP_SOURCES = ["rsd_runtime.hip", "k_small.hip", "k_fourstep.hip", "k_misc.hip"]
+C_SOURCES = ["rawspec_api.c"]
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # packed FP32 instructions stay disabled for all device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    f"-I{INC}", f"-I{CSRC}",
+]
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra", "-Wno-unused-parameter",
+           f"-I{INC}", f"-I{CSRC}"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def _obj(src):
+    return os.path.join(BUILD, os.path.basename(src) + ".o")
+
+
+def compile_one(src):
+    path = os.path.join(CSRC, src)
+    out = _obj(src)
+    if src.endswith(".hip"):
+        cmd = [HIPCC, "-c", *HIP_FLAGS, "-x", "hip", path, "-o", out]
+    else:
+        cmd = [CC, "-c", *C_FLAGS, path, "-o", out]
+    _run(cmd)
+    return out
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> str:
+    """Compile every source and link librawspec.so; returns its path."""
+    if not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = HIP_SOURCES + C_SOURCES
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, srcs))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
+          "-Wl,--no-undefined", "-Wl,-soname,librawspec.so"])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def exported_symbols(path: str = LIB):
+    nm = shutil.which("nm") or "nm"
+    r = _run([nm, "-D", "--defined-only", path])
+    return sorted(line.split()[-1] for line in r.stdout.splitlines() if " T " in line)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--check", action="store_true")
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args(argv)
+    build(a.j, verbose=True)
+    if a.check:
+        for s in exported_symbols():
+            if s.startswith(("rawspec", "get_")):
+                print(s)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

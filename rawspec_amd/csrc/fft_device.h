@@ -1,0 +1,273 @@
+// fft_device.h -- device-side building blocks for the gfx950 spectrometer
+// kernels: radix-2/4/8/16 butterflies, an LDS-exchanged Stockham FFT executed
+// by a "team" of threads, sample unpacking and polarisation detection.
+//
+// Conventions used by every kernel:
+//  * All transforms are computed with the forward kernel exp(-2 pi i k m / N).
+//    The inverse transform rawspec asks for (fft_dir <= 0) is obtained as
+//    conj(FFT(conj(x))): the unpacker negates the imaginary input and the
+//    detector negates the imaginary cross term (powers are unchanged).
+//  * Samples are transformed as raw integers (after the SNORM clamp of the
+//    most negative code); the 1/127^2 or 1/32767^2 normalisation of the
+//    texture read (rawspec_gpu.cu:845-863) is applied once to the detected
+//    sums.
+//  * A team of T threads holds an N = T*E point transform; thread j owns
+//    elements j + m*T (m < E) on entry and frequency bins j + m*T on exit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+
+namespace rsk {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+// Multiplies a by W_R^k = exp(-2 pi i k / R), R <= 16, k compile-time after
+// unrolling.  Exact special cases for multiples of pi/4.
+template <int R>
+__device__ __forceinline__ float2 twc(float2 a, int k) {
+  constexpr float R2 = 0.70710678118654752440f;   // cos(pi/4)
+  constexpr float C8 = 0.92387953251128675613f;   // cos(pi/8)
+  constexpr float S8 = 0.38268343236508977173f;   // sin(pi/8)
+  const int e = (k * (16 / R)) & 15;              // angle in units of 2 pi / 16
+  switch (e) {
+    case 0: return a;
+    case 4: return make_float2(a.y, -a.x);
+    case 8: return make_float2(-a.x, -a.y);
+    case 12: return make_float2(-a.y, a.x);
+    case 2: return make_float2((a.x + a.y) * R2, (a.y - a.x) * R2);
+    case 6: return make_float2((a.y - a.x) * R2, -(a.x + a.y) * R2);
+    case 10: return make_float2((-a.x - a.y) * R2, (a.x - a.y) * R2);
+    case 14: return make_float2((a.x - a.y) * R2, (a.x + a.y) * R2);
+    default: {
+      // general angle: exp(-i t) * (x + i y) = (x c + y s) + i (y c - x s)
+      float c, s;
+      switch (e) {
+        case 1: c = C8; s = S8; break;
+        case 3: c = S8; s = C8; break;
+        case 5: c = -S8; s = C8; break;
+        case 7: c = -C8; s = S8; break;
+        case 9: c = -C8; s = -S8; break;
+        case 11: c = -S8; s = -C8; break;
+        case 13: c = S8; s = -C8; break;
+        default: c = C8; s = -S8; break;   // 15
+      }
+      return make_float2(a.x * c + a.y * s, a.y * c - a.x * s);
+    }
+  }
+}
+
+// In-place forward DFT of R points held in registers (radix-2 recursion,
+// fully unrolled).
+template <int R>
+struct Dft {
+  __device__ __forceinline__ static void run(float2 (&a)[R]) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) { e[k] = a[2 * k]; o[k] = a[2 * k + 1]; }
+    Dft<R / 2>::run(e);
+    Dft<R / 2>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      float2 t = twc<R>(o[k], k);
+      a[k] = cadd(e[k], t);
+      a[k + R / 2] = csub(e[k], t);
+    }
+  }
+};
+template <>
+struct Dft<1> {
+  __device__ __forceinline__ static void run(float2 (&)[1]) {}
+};
+template <>
+struct Dft<2> {
+  __device__ __forceinline__ static void run(float2 (&a)[2]) {
+    float2 t = a[1];
+    a[1] = csub(a[0], t);
+    a[0] = cadd(a[0], t);
+  }
+};
+
+// One Stockham stage of radix R on an N = T*E point team transform, for NP
+// interleaved polarisations.  Butterfly b = j + u*T (u < E/R) reads elements
+// b + r*N/R, which are the thread's own slots u + r*E/R; it writes element r
+// to (b / NS)*NS*R + (b % NS) + r*NS.  The last stage (NS*R == N) writes back
+// into the same register slots, so no exchange follows it.
+template <int N, int E, int NP, int R, int NS, bool LAST>
+__device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
+                                      const float2 *__restrict__ tw) {
+  constexpr int T = N / E;
+  constexpr int NB = E / R;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int b = j + u * T;
+    const int k = b & (NS - 1);
+    float2 w[R];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[r] = tw[(r * k) * (kTwN / (NS * R))];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      float2 a[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[r] = v[p][u + r * NB];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w[r]);
+      }
+      Dft<R>::run(a);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[p][u + r * NB] = a[r];
+      } else {
+        const int base = (b / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[p * N + base + r * NS] = a[r];
+      }
+    }
+  }
+  if constexpr (!LAST) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[p][m] = lds[p * N + j + m * T];
+    __syncthreads();
+  }
+}
+
+template <int N, int E, int NP, int NS>
+struct Stages {
+  __device__ __forceinline__ static void run(float2 (&v)[NP][E], float2 *lds, int j,
+                                             const float2 *__restrict__ tw) {
+    if constexpr (NS * E == N) {
+      stage<N, E, NP, E, NS, true>(v, lds, j, tw);
+    } else {
+      stage<N, E, NP, E, NS, false>(v, lds, j, tw);
+      Stages<N, E, NP, NS * E>::run(v, lds, j, tw);
+    }
+  }
+};
+
+// Forward FFT of N points (power of two, E | N, N <= 4096) for NP
+// polarisations held by a team of N/E threads.  lds: NP*N float2 private to
+// the team.  Every thread of the workgroup must call it (barriers inside).
+template <int N, int E, int NP>
+__device__ __forceinline__ void team_fft(float2 (&v)[NP][E], float2 *lds, int j,
+                                         const float2 *__restrict__ tw) {
+  if constexpr (N > 1) {
+    constexpr int R0 = 1 << (ilog2(N) % ilog2(E));
+    if constexpr (R0 > 1) {
+      if constexpr (R0 == N) {
+        stage<N, E, NP, R0, 1, true>(v, lds, j, tw);
+      } else {
+        stage<N, E, NP, R0, 1, false>(v, lds, j, tw);
+        Stages<N, E, NP, R0>::run(v, lds, j, tw);
+      }
+    } else {
+      Stages<N, E, NP, 1>::run(v, lds, j, tw);
+    }
+  }
+}
+
+// Elements per thread for a team transform of N points.
+constexpr int elems_for(int n) {
+  return n <= 16 ? n : (n <= 128 ? 8 : 16);
+}
+
+// ---- sample unpacking ----------------------------------------------------
+// Raw sample of NP pols with NBPS-bit components -> NP complex floats
+// (integer valued), most negative code clamped to -(2^(NBPS-1)-1) as the
+// SNORM read does, imaginary part negated when inv != 0.
+template <int NBPS, int NP>
+struct Sample;
+
+template <>
+struct Sample<8, 2> {
+  using word = uint32_t;
+  __device__ __forceinline__ static void cvt(word w, float2 (&o)[2], float isign) {
+    int a = (int)(int8_t)(w & 0xff), b = (int)(int8_t)((w >> 8) & 0xff);
+    int c = (int)(int8_t)((w >> 16) & 0xff), d = (int)(int8_t)(w >> 24);
+    a = max(a, -127); b = max(b, -127); c = max(c, -127); d = max(d, -127);
+    o[0] = make_float2((float)a, isign * (float)b);
+    o[1] = make_float2((float)c, isign * (float)d);
+  }
+};
+template <>
+struct Sample<8, 1> {
+  using word = uint16_t;
+  __device__ __forceinline__ static void cvt(word w, float2 (&o)[1], float isign) {
+    int a = (int)(int8_t)(w & 0xff), b = (int)(int8_t)(w >> 8);
+    a = max(a, -127); b = max(b, -127);
+    o[0] = make_float2((float)a, isign * (float)b);
+  }
+};
+template <>
+struct Sample<16, 2> {
+  using word = uint2;
+  __device__ __forceinline__ static void cvt(word w, float2 (&o)[2], float isign) {
+    int a = (int)(int16_t)(w.x & 0xffff), b = (int)(int16_t)(w.x >> 16);
+    int c = (int)(int16_t)(w.y & 0xffff), d = (int)(int16_t)(w.y >> 16);
+    a = max(a, -32767); b = max(b, -32767); c = max(c, -32767); d = max(d, -32767);
+    o[0] = make_float2((float)a, isign * (float)b);
+    o[1] = make_float2((float)c, isign * (float)d);
+  }
+};
+template <>
+struct Sample<16, 1> {
+  using word = uint32_t;
+  __device__ __forceinline__ static void cvt(word w, float2 (&o)[1], float isign) {
+    int a = (int)(int16_t)(w & 0xffff), b = (int)(int16_t)(w >> 16);
+    a = max(a, -32767); b = max(b, -32767);
+    o[0] = make_float2((float)a, isign * (float)b);
+  }
+};
+
+// ---- detection -------------------------------------------------------------
+// PMODE 1: total power (sum over pols); 4: XX, YY, Re XY*, Im XY*;
+// -4: I, Q, U, V.  gam = +-1 is the sign applied to the imaginary cross term
+// (input_conjugated and the inverse-via-conjugation trick combined).
+template <int PMODE>
+constexpr int nq() { return PMODE == 1 ? 1 : 4; }
+
+template <int PMODE, int NP>
+__device__ __forceinline__ void detect_add(float (&acc)[nq<PMODE>()], const float2 (&x)[NP],
+                                           float gam) {
+  const float p0 = x[0].x * x[0].x + x[0].y * x[0].y;
+  if constexpr (PMODE == 1) {
+    if constexpr (NP == 2) acc[0] += p0 + (x[1].x * x[1].x + x[1].y * x[1].y);
+    else acc[0] += p0;
+  } else {
+    const float p1 = x[1].x * x[1].x + x[1].y * x[1].y;
+    const float re = x[0].x * x[1].x + x[0].y * x[1].y;
+    const float im = gam * (x[0].y * x[1].x - x[0].x * x[1].y);
+    if constexpr (PMODE == 4) {
+      acc[0] += p0;
+      acc[1] += p1;
+    } else {
+      acc[0] += p0 + p1;
+      acc[1] += p0 - p1;
+    }
+    acc[2] += re;
+    acc[3] += im;
+  }
+}
+
+// Bijective blockIdx -> logical tile map that hands each of the 8 XCDs a
+// contiguous range of tiles (hardware dispatches workgroups round-robin over
+// XCDs), so neighbouring tiles share one L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned bid, unsigned nb) {
+  const unsigned q = nb >> 3, r = nb & 7, x = bid & 7, i = bid >> 3;
+  return x * q + min(x, r) + i;
+}
+
+}  // namespace rsk

@@ -1,0 +1,202 @@
+// k_fourstep.hip -- spectrometer kernels for FFT lengths 8192 .. 2^24.
+//
+// Bailey's four-step decomposition N = N1*N2 (n = n1 + N1*n2,
+// k = k2 + N2*k1):
+//   pass A  Y[n1][k2] = W_N^(n1*k2) * sum_n2 x[n1 + N1*n2] W_N2^(n2*k2)
+//   pass B  X[k2 + N2*k1] = sum_n1 Y[n1][k2] W_N1^(n1*k1)
+// Pass A fuses the int8/int16 unpack into its first radix stage; pass B fuses
+// detection, integration of the dump group's spectra and the FFT-shift
+// (k -> k + N/2 becomes k1 -> k1 + N1/2).  Together they replace the
+// reference's multi-pass cuFFT plan with load/store callbacks
+// (rawspec_gpu.cu:133-267, 1182-1195, 1555-1569) plus its accumulate kernel
+// and shift copies (:297-318, :1647-1697).  Workgroup ids are remapped so each
+// XCD processes a contiguous range of tiles and shares cache lines in its L2.
+#include "fft_device.h"
+#include "kernels.h"
+
+namespace rsk {
+
+constexpr int kWG = 256;
+
+template <int N2, int NBPS, int NP>
+__global__ __launch_bounds__(kWG) void k_fs_a(FsArgs a) {
+  constexpr int E = elems_for(N2);
+  constexpr int T = N2 / E;
+  constexpr int TA = kWG / T;          // n1 values per workgroup
+  using S = Sample<NBPS, NP>;
+  using W = typename S::word;
+  __shared__ float2 lds[TA * NP * N2 > 1 ? TA * NP * N2 : 1];
+
+  const unsigned N1 = a.N1;
+  const unsigned long long N = (unsigned long long)N1 * N2;
+  const unsigned tiles = N1 / TA;
+  const unsigned nb = gridDim.x;
+  const unsigned l = xcd_remap(blockIdx.x, nb);
+  const unsigned tile = l % tiles;
+  const unsigned cs = l / tiles;
+  const unsigned sp = cs % a.Nss;
+  const unsigned c = cs / a.Nss;
+  const int tid = threadIdx.x;
+  const int sg = tid % TA;             // n1 slot: fastest across lanes
+  const int j = tid / TA;
+  const unsigned n1 = tile * TA + sg;
+
+  const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)c * a.chan_stride) +
+                 (unsigned long long)sp * N + n1;
+  float2 v[NP][E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const W w = src[(unsigned long long)(j + m * T) * N1];
+    float2 o[NP];
+    S::cvt(w, o, a.isign);
+#pragma unroll
+    for (int p = 0; p < NP; ++p) v[p][m] = o[p];
+  }
+  team_fft<N2, E, NP>(v, lds + sg * NP * N2, j, a.tw);
+
+  float2 *dst = a.work + ((((unsigned long long)c * a.Nss + sp) * N1 + n1) * N2) * NP;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const unsigned k2 = j + m * T;
+    const unsigned e = n1 * k2;        // < N <= 2^24
+    const float2 w = cmul(a.twc[e >> kTwLog], a.twf[e & (kTwN - 1)]);
+    if constexpr (NP == 2) {
+      const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
+      reinterpret_cast<float4 *>(dst)[k2] = make_float4(y0.x, y0.y, y1.x, y1.y);
+    } else {
+      dst[k2] = cmul(v[0][m], w);
+    }
+  }
+}
+
+template <int N1, int NP, int PMODE>
+__global__ __launch_bounds__(kWG) void k_fs_b(FsArgs a) {
+  constexpr int E = elems_for(N1);
+  constexpr int T = N1 / E;
+  constexpr int TB = kWG / T;          // k2 values per workgroup
+  constexpr int Q = nq<PMODE>();
+  __shared__ float2 lds[TB * NP * N1];
+
+  const unsigned N2 = a.N2;
+  const unsigned long long N = (unsigned long long)N1 * N2;
+  const unsigned tiles = N2 / TB;
+  const unsigned nb = gridDim.x;
+  const unsigned l = xcd_remap(blockIdx.x, nb);
+  const unsigned tile = l % tiles;
+  const unsigned cg = l / tiles;
+  const unsigned g = cg % a.Ng;
+  const unsigned c = cg / a.Ng;
+  const int tid = threadIdx.x;
+  const int sg = tid % TB;             // k2 slot: fastest across lanes
+  const int j = tid / TB;
+  const unsigned k2 = tile * TB + sg;
+
+  float acc[E][Q];
+#pragma unroll
+  for (int m = 0; m < E; ++m)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[m][q] = 0.0f;
+
+  for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
+    const float2 *y = a.work + ((((unsigned long long)c * a.Nss + sp) * N1) * N2 + k2) * NP;
+    float2 v[NP][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const unsigned long long off = (unsigned long long)(j + m * T) * N2 * NP;
+      if constexpr (NP == 2) {
+        const float4 t = *reinterpret_cast<const float4 *>(y + off);
+        v[0][m] = make_float2(t.x, t.y);
+        v[1][m] = make_float2(t.z, t.w);
+      } else {
+        v[0][m] = y[off];
+      }
+    }
+    team_fft<N1, E, NP>(v, lds + sg * NP * N1, j, a.tw);
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float2 x[NP];
+#pragma unroll
+      for (int p = 0; p < NP; ++p) x[p] = v[p][m];
+      detect_add<PMODE, NP>(acc[m], x, a.gam);
+    }
+  }
+
+  float *out = a.acc + ((unsigned long long)g * Q * a.Nc + c) * N;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const unsigned k1 = j + m * T;
+    const unsigned long long jo = k2 + (unsigned long long)N2 * ((k1 + N1 / 2) & (N1 - 1));
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      float *o = out + (unsigned long long)q * a.Nc * N + jo;
+      const float val = acc[m][q] * a.scale;
+      *o = a.first ? val : (*o + val);
+    }
+  }
+}
+
+int fs_tile_a(int n2) { return kWG / (n2 / elems_for(n2)); }
+int fs_tile_b(int n1) { return kWG / (n1 / elems_for(n1)); }
+
+template <int N2, int NBPS>
+static int go_a(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
+  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, NBPS, 2>), dim3(nb), dim3(kWG), 0, s, a);
+  else hipLaunchKernelGGL((k_fs_a<N2, NBPS, 1>), dim3(nb), dim3(kWG), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+template <int NBPS>
+static int go_a_n(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
+  switch (a.N2) {
+    case 64: return go_a<64, NBPS>(np, a, nb, s);
+    case 128: return go_a<128, NBPS>(np, a, nb, s);
+    case 256: return go_a<256, NBPS>(np, a, nb, s);
+    case 512: return go_a<512, NBPS>(np, a, nb, s);
+    case 1024: return go_a<1024, NBPS>(np, a, nb, s);
+    case 2048: return go_a<2048, NBPS>(np, a, nb, s);
+    case 4096: return go_a<4096, NBPS>(np, a, nb, s);
+  }
+  return 2;
+}
+
+int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
+  const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
+  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.Nc;
+  if (nb == 0 || nb > 0x7fffffffull) return 2;
+  if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);
+  if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, s);
+  return 2;
+}
+
+template <int N1>
+static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) {
+  if (np == 1) {
+    hipLaunchKernelGGL((k_fs_b<N1, 1, 1>), dim3(nb), dim3(kWG), 0, s, a);
+  } else if (pmode == 1) {
+    hipLaunchKernelGGL((k_fs_b<N1, 2, 1>), dim3(nb), dim3(kWG), 0, s, a);
+  } else if (pmode == 4) {
+    hipLaunchKernelGGL((k_fs_b<N1, 2, 4>), dim3(nb), dim3(kWG), 0, s, a);
+  } else if (pmode == -4) {
+    hipLaunchKernelGGL((k_fs_b<N1, 2, -4>), dim3(nb), dim3(kWG), 0, s, a);
+  } else {
+    return 2;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
+  const unsigned tiles = a.N2 / fs_tile_b((int)a.N1);
+  const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.Nc;
+  if (nb == 0 || nb > 0x7fffffffull) return 2;
+  switch (a.N1) {
+    case 128: return go_b<128>(np, pmode, a, (unsigned)nb, s);
+    case 256: return go_b<256>(np, pmode, a, (unsigned)nb, s);
+    case 512: return go_b<512>(np, pmode, a, (unsigned)nb, s);
+    case 1024: return go_b<1024>(np, pmode, a, (unsigned)nb, s);
+    case 2048: return go_b<2048>(np, pmode, a, (unsigned)nb, s);
+    case 4096: return go_b<4096>(np, pmode, a, (unsigned)nb, s);
+  }
+  return 2;
+}
+
+}  // namespace rsk

@@ -1,0 +1,88 @@
+// kernels.h -- launch interface of the spectrometer kernels (internal to
+// librawspec; the host C layer reaches them only through rsd.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rsk {
+
+constexpr int kTwLog = 12;            // master twiddle table: W_4096^m
+constexpr int kTwN = 1 << kTwLog;     // also the largest single-pass FFT
+
+// Fused small-N path (N <= 4096): unpack -> FFT -> detect -> integrate a
+// chunk of `chunk` spectra.  Workgroup b covers channel b / bpc and either
+// groups [gpw*(b % bpc), +gpw) whole, or chunk (b % bpc) % G of group
+// (b % bpc) / G.
+struct SmallArgs {
+  const uint8_t *in;              // device input buffer [Nc][Nb*Ntpb][Np][2]
+  unsigned long long chan_stride; // bytes per channel of the input buffer
+  unsigned Nc;
+  unsigned na_buf;                // spectra per dump group in this buffer
+  unsigned chunk;                 // spectra per workgroup
+  unsigned G;                     // chunks per group (gpw == 1 when G > 1)
+  unsigned Ng;                    // groups per channel
+  unsigned gpw;                   // groups per workgroup
+  unsigned spg;                   // concurrent slots per group (SLOTS / gpw)
+  unsigned bpc;                   // workgroups per channel: ceil(Ng/gpw)*G
+  float isign;                    // -1: negate imaginary input (inverse FFT)
+  float gam;                      // sign of the imaginary cross term
+  float scale;                    // SNORM normalisation of the detected sums
+  int direct;                     // 1: write the integration buffer directly
+  int first;                      // direct: overwrite (1) or accumulate (0)
+  float *acc;                     // [Nds][Q][Nc][N], FFT-shifted
+  float *part;                    // [Nc][Ng][G][Q][N] unscaled, FFT-shifted
+  const float2 *tw;               // W_4096^m, m < 4096
+};
+
+// Four-step path (N = N1*N2 > 4096).  Pass A: length-N2 FFTs over stride N1
+// input, twiddle W_N^(n1*k2), store Y[n1][k2][pol].  Pass B: length-N1 FFTs
+// per k2, detect, integrate the group's spectra, write FFT-shifted output.
+struct FsArgs {
+  const uint8_t *in;
+  unsigned long long chan_stride;
+  unsigned Nc, Nss, na_buf, Ng;
+  unsigned N1, N2;
+  float isign, gam, scale;
+  int first;
+  float2 *work;                   // [Nc][Nss][N1][N2][Np]
+  float *acc;
+  const float2 *tw;               // W_4096^m
+  const float2 *twf;              // W_N^i, i < 4096
+  const float2 *twc;              // W_N^(4096 h), h < N/4096
+};
+
+struct ReduceArgs {
+  const float *part;
+  float *acc;
+  unsigned Nc, Ng, G, Q, N;
+  float scale;
+  int first;
+};
+
+struct IcsArgs {
+  const float *acc;
+  float *ics;
+  const float *w;
+  unsigned Nc, Nant, Ncpa, Q, N, Nds;
+};
+
+// Query functions used by the host-side planner.
+int small_wg(int n);            // threads per workgroup of the small kernel
+int small_slots(int n);         // spectra processed concurrently per workgroup
+int fs_tile_a(int n2);          // n1 values per pass-A workgroup
+int fs_tile_b(int n1);          // k2 values per pass-B workgroup
+
+// Launchers; return 0 on success, non-zero for an unsupported shape or a
+// launch error.
+int launch_small(int n, int nbps, int np, int pmode, const SmallArgs &a,
+                 unsigned nblocks, hipStream_t s);
+int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s);
+int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
+int launch_reduce(const ReduceArgs &a, hipStream_t s);
+int launch_ics(const IcsArgs &a, hipStream_t s);
+int launch_expand4(const uint8_t *stage, uint8_t *dst, unsigned Nc,
+                   unsigned long long chan_bytes4, unsigned long long dst_chan_stride,
+                   unsigned long long dst_block_off, hipStream_t s);
+
+}  // namespace rsk

@@ -1,0 +1,513 @@
+/*
+ * rawspec_api.c -- librawspec's public C API (include/rawspec.h) on top of
+ * the gfx950 device layer (rsd.h).
+ *
+ * This file owns the semantics a drop-in replacement must honour:
+ * validation and derived counts (rawspec_gpu.cu:471-641, 732-760), buffer
+ * ownership (:694-782, :1300-1391), the per-buffer dump schedule and the
+ * PRE/POST callback order (:1517-1747), and completion/reset behaviour
+ * (:1762-1836).  It never calls HIP directly.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rawspec.h"
+#include "rawspec_amd.h"
+#include "rsd.h"
+
+#define RS_ABS(x) ((x) < 0 ? -(x) : (x))
+
+#ifndef RAWSPEC_AMD_VERSION
+#define RAWSPEC_AMD_VERSION "3.0-amd.1"
+#endif
+
+typedef struct {
+  rawspec_context *ctx;
+  int output_product;
+  int type;                       /* RAWSPEC_CALLBACK_PRE/POST_DUMP */
+} rs_cb_t;
+
+typedef struct {
+  rsd_ctx *dev;
+  unsigned int inbuf_count;       /* buffers processed since init/reset */
+  int caller_managed;             /* h_blkbufs belong to the caller      */
+  int expand4;                    /* Nbps was 4                          */
+  int device_output;              /* dumps stay in device memory         */
+  unsigned int Nss[MAX_OUTPUTS];
+  unsigned int Nis[MAX_OUTPUTS];
+  rs_cb_t pre[MAX_OUTPUTS];
+  rs_cb_t post[MAX_OUTPUTS];
+} rs_gpu_ctx;
+
+static void rs_err(const char *msg)
+{
+  fprintf(stderr, "librawspec: %s\n", msg);
+  fflush(stderr);
+}
+
+const char *get_librawspec_version()
+{
+  return RAWSPEC_AMD_VERSION;
+}
+
+const char *get_cufft_version()
+{
+  return rsd_engine_name();
+}
+
+/* Runs the client callback on the stream's host-function thread
+ * (rawspec_gpu.cu:378-402). */
+static void rs_cb_trampoline(void *arg)
+{
+  rs_cb_t *cb = (rs_cb_t *)arg;
+  if (cb->ctx->dump_callback)
+    cb->ctx->dump_callback(cb->ctx, cb->output_product, cb->type);
+}
+
+static unsigned long rs_mod(off_t x, unsigned long n)
+{
+  long long r = (long long)x % (long long)n;
+  return (unsigned long)(r < 0 ? r + (long long)n : r);
+}
+
+int rawspec_initialize(rawspec_context *ctx)
+{
+  unsigned int i;
+  int expand4;
+  rs_gpu_ctx *g;
+  rsd_config cfg;
+  float *weights = NULL;
+  unsigned long long nbuf;
+
+  /* ---- validation, same rules and order as rawspec_gpu.cu:471-641 ---- */
+  if (ctx->No == 0 || ctx->No > MAX_OUTPUTS) {
+    fprintf(stderr, "output products must be in range [1..%d], not %d\n", MAX_OUTPUTS, ctx->No);
+    fflush(stderr);
+    return 1;
+  }
+  if (ctx->Np == 0 || ctx->Np > 2) {
+    fprintf(stderr, "number of polarizations must be in range [1..2], not %d\n", ctx->Np);
+    fflush(stderr);
+    return 1;
+  }
+  for (i = 0; i < ctx->No; i++)
+    if (RS_ABS(ctx->Npolout[i]) != 4 || ctx->Np != 2) ctx->Npolout[i] = 1;
+  if (ctx->Ntpb == 0) {
+    rs_err("number of time samples per block cannot be zero");
+    return 1;
+  }
+  if (ctx->Nbps == 0) ctx->Nbps = 8;
+  if (ctx->Nbps != 4 && ctx->Nbps != 8 && ctx->Nbps != 16) {
+    fprintf(stderr, "Number of bits per sample in raw header must be 4, 0/8, or 16\n"
+                    "Observed a value of %d\n", ctx->Nbps);
+    fflush(stderr);
+    return 1;
+  }
+  expand4 = ctx->Nbps == 4;
+  if (ctx->Nbps == 4) ctx->Nbps = 8;
+  ctx->Ntmax = 0;
+  for (i = 0; i < ctx->No; i++) {
+    if (ctx->Nts[i] == 0) {
+      fprintf(stderr, "Nts[%d] cannot be 0\n", i);
+      fflush(stderr);
+      return 1;
+    }
+    if (ctx->Ntmax < ctx->Nts[i]) ctx->Ntmax = ctx->Nts[i];
+  }
+  for (i = 0; i < ctx->No; i++) {
+    if (ctx->Ntmax % ctx->Nts[i] != 0) {
+      fprintf(stderr, "Nts[%d] (%u) is not a factor of Ntmax (%u)\n", i, ctx->Nts[i], ctx->Ntmax);
+      fflush(stderr);
+      return 1;
+    }
+  }
+  if (ctx->Nb != 0) {
+    if (((unsigned long long)ctx->Nb * ctx->Ntpb) % ctx->Ntmax != 0) {
+      fprintf(stderr, "Ntmax (%u) is not a factor of Nb*Ntpb (%u * %u)\n", ctx->Ntmax, ctx->Nb,
+              ctx->Ntpb);
+      fflush(stderr);
+      return 1;
+    }
+  } else if (ctx->Ntmax < ctx->Ntpb) {
+    if (ctx->Ntpb % ctx->Ntmax != 0) {
+      fprintf(stderr, "Ntmax (%u) is not a factor of Ntpb (%u)\n", ctx->Ntmax, ctx->Ntpb);
+      fflush(stderr);
+      return 1;
+    }
+    ctx->Nb = 1;
+  } else {
+    if (ctx->Ntmax % ctx->Ntpb != 0) {
+      fprintf(stderr, "Ntpb (%u) is not a factor of Nmax (%u)\n", ctx->Ntpb, ctx->Ntmax);
+      fflush(stderr);
+      return 1;
+    }
+    ctx->Nb = ctx->Ntmax / ctx->Ntpb;
+  }
+  if (ctx->Nb_host == 0 && ctx->h_blkbufs) {
+    rs_err("Must specify number of host input blocks when caller-managed");
+    return 1;
+  } else if (ctx->Nb_host == 0) {
+    ctx->Nb_host = ctx->Nb;
+  }
+  nbuf = (unsigned long long)ctx->Nb * ctx->Ntpb;
+  for (i = 0; i < ctx->No; i++) {
+    unsigned long long ntna = (unsigned long long)ctx->Nts[i] * ctx->Nas[i];
+    if (ctx->Nas[i] == 0) {
+      fprintf(stderr, "Nas[%d] cannot be 0\n", i);
+      fflush(stderr);
+      return 1;
+    }
+    if (ntna < nbuf ? (nbuf % ntna != 0) : (ntna % nbuf != 0)) {
+      fprintf(stderr, "Nts[%d] * Nas[%d] (%u * %u) and Nb * Ntpb (%u * %u) must divide one another\n",
+              i, i, ctx->Nts[i], ctx->Nas[i], ctx->Nb, ctx->Ntpb);
+      fflush(stderr);
+      return 1;
+    }
+  }
+  if (nbuf > 0xffffffffull) {
+    rs_err("Nb * Ntpb exceeds 2^32 samples per channel");
+    return 1;
+  }
+  if (ctx->Nant == 0) ctx->Nant = 1;
+  if (ctx->Nbc == 0) {
+    ctx->Nbc = ctx->Nc;
+  } else {
+    if (ctx->Nbc <= 1) {
+      if (ctx->Nant > 1) {
+        ctx->Nbc = ctx->Nc / ctx->Nant;
+      } else {
+        for (i = 1; i <= 10; i++)
+          if (ctx->Nc % i == 0) ctx->Nbc = ctx->Nc / i;
+      }
+    }
+    if (ctx->Nbc == 0 || ctx->Nc % ctx->Nbc != 0) {
+      fprintf(stderr, "%d channels cannot be factorised to batches of %d\n", ctx->Nc, ctx->Nbc);
+      fflush(stderr);
+      return 1;
+    }
+    printf("Batching %d channels into %d batches of %d.\n", ctx->Nc, ctx->Nc / ctx->Nbc, ctx->Nbc);
+  }
+  if (ctx->incoherently_sum) {
+    if (!((ctx->Naws == 1 && (unsigned)ctx->Naws < ctx->Nant) ||
+          (ctx->Naws > 0 && (unsigned)ctx->Naws == ctx->Nant)) || !ctx->Aws) {
+      fprintf(stderr, "Not enough antenna-weights provided for the %d antennas: only provided %d.\n",
+              ctx->Nant, ctx->Naws);
+      fflush(stderr);
+      return 1;
+    }
+  }
+
+  /* ---- allocation (rawspec_gpu.cu:643-782) ---- */
+  for (i = 0; i < MAX_OUTPUTS; i++) {
+    ctx->h_pwrbuf[i] = NULL;
+    ctx->h_icsbuf[i] = NULL;
+  }
+  ctx->gpu_ctx = NULL;
+
+  g = (rs_gpu_ctx *)calloc(1, sizeof(*g));
+  if (!g) {
+    rs_err("unable to allocate the context");
+    return 1;
+  }
+  ctx->gpu_ctx = g;
+  g->expand4 = expand4;
+  for (i = 0; i < MAX_OUTPUTS; i++) {
+    g->pre[i].ctx = ctx;
+    g->pre[i].output_product = (int)i;
+    g->pre[i].type = RAWSPEC_CALLBACK_PRE_DUMP;
+    g->post[i].ctx = ctx;
+    g->post[i].output_product = (int)i;
+    g->post[i].type = RAWSPEC_CALLBACK_POST_DUMP;
+  }
+
+  if (!ctx->h_blkbufs) {
+    size_t bsz = (size_t)ctx->Ntpb * ctx->Np * ctx->Nc * 2 * (ctx->Nbps / 8);
+    g->caller_managed = 0;
+    ctx->h_blkbufs = (char **)calloc(ctx->Nb_host, sizeof(char *));
+    if (!ctx->h_blkbufs) goto fail;
+    for (i = 0; i < ctx->Nb_host; i++)
+      if (rsd_host_alloc(ctx->gpu_index, (void **)&ctx->h_blkbufs[i], bsz, 1)) goto fail;
+  } else {
+    g->caller_managed = 1;
+  }
+
+  for (i = 0; i < ctx->No; i++) {
+    g->Nss[i] = (unsigned)(nbuf / ctx->Nts[i]);
+    ctx->Nds[i] = g->Nss[i] / ctx->Nas[i];
+    if (ctx->Nds[i] == 0) ctx->Nds[i] = 1;
+    g->Nis[i] = ctx->Nas[i] / g->Nss[i];
+    if (g->Nis[i] == 0) g->Nis[i] = 1;
+    ctx->h_pwrbuf_size[i] = (size_t)RS_ABS(ctx->Npolout[i]) * ctx->Nds[i] * ctx->Nts[i] *
+                            ctx->Nc * sizeof(float);
+    if (rsd_host_alloc(ctx->gpu_index, (void **)&ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i], 0))
+      goto fail;
+    if (ctx->incoherently_sum == 1) {
+      if (rsd_host_alloc(ctx->gpu_index, (void **)&ctx->h_icsbuf[i],
+                         ctx->h_pwrbuf_size[i] / ctx->Nant, 0))
+        goto fail;
+    }
+  }
+
+  memset(&cfg, 0, sizeof(cfg));
+  cfg.gpu_index = ctx->gpu_index;
+  cfg.Nc = ctx->Nc;
+  cfg.Np = ctx->Np;
+  cfg.Ntpb = ctx->Ntpb;
+  cfg.Nb = ctx->Nb;
+  cfg.Nbps = ctx->Nbps;
+  cfg.expand4 = expand4;
+  cfg.Nant = ctx->Nant;
+  cfg.conj = ctx->input_conjugated ? 1 : 0;
+  cfg.ics = ctx->incoherently_sum ? 1 : 0;
+  cfg.No = ctx->No;
+  if (cfg.ics) {
+    /* one broadcast weight or one per antenna (rawspec_gpu.cu:1040-1055) */
+    weights = (float *)malloc(ctx->Nant * sizeof(float));
+    if (!weights) goto fail;
+    for (i = 0; i < ctx->Nant; i++) weights[i] = ctx->Naws == 1 ? ctx->Aws[0] : ctx->Aws[i];
+    cfg.weights = weights;
+  }
+  for (i = 0; i < ctx->No; i++) {
+    cfg.prod[i].Nt = ctx->Nts[i];
+    cfg.prod[i].Na = ctx->Nas[i];
+    cfg.prod[i].Nss = g->Nss[i];
+    cfg.prod[i].Nds = ctx->Nds[i];
+    cfg.prod[i].Nis = g->Nis[i];
+    cfg.prod[i].npolout = ctx->Npolout[i];
+  }
+  if (rsd_create(&cfg, &g->dev)) goto fail;
+  free(weights);
+  return 0;
+
+fail:
+  free(weights);
+  rawspec_cleanup(ctx);
+  return 1;
+}
+
+void rawspec_cleanup(rawspec_context *ctx)
+{
+  unsigned int i;
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+
+  /* finish outstanding work before host buffers disappear under it */
+  if (g && g->dev) rsd_sync(g->dev);
+  for (i = 0; i < MAX_OUTPUTS; i++) {
+    if (ctx->h_pwrbuf[i]) {
+      rsd_host_free(ctx->h_pwrbuf[i]);
+      ctx->h_pwrbuf[i] = NULL;
+    }
+    if (ctx->h_icsbuf[i]) {
+      rsd_host_free(ctx->h_icsbuf[i]);
+      ctx->h_icsbuf[i] = NULL;
+    }
+  }
+  if (g) {
+    if (!g->caller_managed && ctx->h_blkbufs) {
+      for (i = 0; i < ctx->Nb_host; i++)
+        if (ctx->h_blkbufs[i]) rsd_host_free(ctx->h_blkbufs[i]);
+      free(ctx->h_blkbufs);
+      ctx->h_blkbufs = NULL;
+    }
+    if (g->dev) rsd_destroy(g->dev);
+    free(g);
+    ctx->gpu_ctx = NULL;
+  }
+}
+
+static rs_gpu_ctx *rs_get(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  if (!g || !g->dev) {
+    rs_err("context is not initialised");
+    return NULL;
+  }
+  return g;
+}
+
+int rawspec_copy_blocks_to_gpu(rawspec_context *ctx, off_t src_idx, off_t dst_idx,
+                               size_t num_blocks)
+{
+  size_t b;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  for (b = 0; b < num_blocks; b++) {
+    unsigned long sblk = rs_mod(src_idx + (off_t)b, ctx->Nb_host);
+    unsigned long dblk = rs_mod(dst_idx + (off_t)b, ctx->Nb);
+    if (rsd_copy_block(g->dev, ctx->h_blkbufs[sblk], (unsigned)dblk)) return 1;
+  }
+  return 0;
+}
+
+/* Host blocks hold complex4 bytes (half the 8-bit block size); each is
+ * expanded on the device into device block (dst_idx + b) % Nb.  The
+ * reference expands into positions that assume num_blocks == Nb and
+ * dst_idx == 0 (rawspec_gpu.cu:1426-1434), the only way its CLI calls it;
+ * this implementation places every block at its own slot. */
+int rawspec_copy_blocks_to_gpu_expanding_complex4(rawspec_context *ctx, off_t src_idx,
+                                                  off_t dst_idx, size_t num_blocks)
+{
+  size_t b;
+  rs_gpu_ctx *g;
+  if (num_blocks > ctx->Nb) {
+    fprintf(stderr, "%s: num_blocks (%lu) > Nb (%u)\n", __func__, (unsigned long)num_blocks,
+            ctx->Nb);
+    fflush(stderr);
+    return 1;
+  }
+  g = rs_get(ctx);
+  if (!g) return 1;
+  if (!g->expand4) {
+    rs_err("complex4 expansion requested but the context was not initialised with Nbps = 4");
+    return 1;
+  }
+  for (b = 0; b < num_blocks; b++) {
+    unsigned long sblk = rs_mod(src_idx + (off_t)b, ctx->Nb_host);
+    unsigned long dblk = rs_mod(dst_idx + (off_t)b, ctx->Nb);
+    if (rsd_copy_block4(g->dev, ctx->h_blkbufs[sblk], (unsigned)dblk)) return 1;
+  }
+  return 0;
+}
+
+int rawspec_zero_blocks_to_gpu(rawspec_context *ctx, off_t dst_idx, size_t num_blocks)
+{
+  size_t b;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  for (b = 0; b < num_blocks; b++) {
+    unsigned long dblk = rs_mod(dst_idx + (off_t)b, ctx->Nb);
+    if (rsd_zero_block(g->dev, (unsigned)dblk)) return 1;
+  }
+  return 0;
+}
+
+/* One buffer through every product, in product order; a product whose
+ * integration completes is dumped between PRE and POST
+ * (rawspec_gpu.cu:1543-1747).  PRE is enqueued before any copy into the
+ * host buffers, so a client that joins its output thread in PRE never races
+ * the copy. */
+static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
+{
+  unsigned int i;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  g->inbuf_count++;
+  for (i = 0; i < ctx->No; i++) {
+    int first = ((g->inbuf_count - 1) % g->Nis[i]) == 0;
+    if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first)) {
+      fprintf(stderr, "librawspec: processing output product %u failed\n", i);
+      fflush(stderr);
+      return 1;
+    }
+    if (g->inbuf_count % g->Nis[i] == 0) {
+      if (ctx->incoherently_sum && rsd_run_ics(g->dev, (int)i)) return 1;
+      if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
+      if (!g->device_output) {
+        if (rsd_dump_to_host(g->dev, (int)i, ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i],
+                             ctx->incoherently_sum ? ctx->h_icsbuf[i] : NULL,
+                             ctx->h_pwrbuf_size[i] / ctx->Nant))
+          return 1;
+      }
+      if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->post[i])) return 1;
+    }
+  }
+  return 0;
+}
+
+int rawspec_start_processing(rawspec_context *ctx, int fft_dir)
+{
+  return rs_start(ctx, NULL, fft_dir);
+}
+
+int rawspec_copy_blocks_to_gpu_and_start_processing(rawspec_context *ctx, size_t num_blocks,
+                                                    char expand4bps_to8bps, int fft_dir)
+{
+  int rc = expand4bps_to8bps ? rawspec_copy_blocks_to_gpu_expanding_complex4(ctx, 0, 0, num_blocks)
+                             : rawspec_copy_blocks_to_gpu(ctx, 0, 0, num_blocks);
+  if (rc) return rc;
+  return rawspec_start_processing(ctx, fft_dir);
+}
+
+int rawspec_reset_integration(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  if (!g || !g->dev) return 1;
+  rawspec_wait_for_completion(ctx);
+  /* The next buffer overwrites (rather than adds to) every integration
+   * buffer, so dropping the partial integrations is just the count. */
+  g->inbuf_count = 0;
+  return 0;
+}
+
+unsigned int rawspec_check_for_completion(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  if (!g || !g->dev) return 0;
+  return rsd_query(g->dev) == 1 ? ctx->No : 0;
+}
+
+int rawspec_wait_for_completion(rawspec_context *ctx)
+{
+  unsigned int i;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  for (i = 0; i < ctx->No; i++)
+    if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
+  if (rsd_sync(g->dev)) return 1;
+  if (*(volatile unsigned int *)&ctx->exit_soon)
+    fprintf(stderr, "*** rawspec_wait_for_completion detected exit_soon enabled.\n"
+                    "*** Cannot continue!\n");
+  return (int)*(volatile unsigned int *)&ctx->exit_soon;
+}
+
+/* ---- extensions (rawspec_amd.h) ---------------------------------------- */
+
+void *rawspec_amd_device_input(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  return (g && g->dev) ? rsd_input_ptr(g->dev) : NULL;
+}
+
+size_t rawspec_amd_device_input_size(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  return (g && g->dev) ? rsd_input_bytes(g->dev) : 0;
+}
+
+int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in, int fft_dir)
+{
+  if (!d_in) {
+    rs_err("rawspec_amd_start_processing_device: NULL input");
+    return 1;
+  }
+  return rs_start(ctx, d_in, fft_dir);
+}
+
+int rawspec_amd_set_device_output(rawspec_context *ctx, int enable)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  g->device_output = enable ? 1 : 0;
+  return 0;
+}
+
+float *rawspec_amd_device_pwrbuf(rawspec_context *ctx, int i)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  if (!g || !g->dev || i < 0 || (unsigned)i >= ctx->No) return NULL;
+  return rsd_pwr_ptr(g->dev, i);
+}
+
+float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  if (!g || !g->dev || i < 0 || (unsigned)i >= ctx->No) return NULL;
+  return rsd_ics_ptr(g->dev, i);
+}
+
+void *rawspec_amd_stream(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  return (g && g->dev) ? rsd_stream(g->dev) : NULL;
+}

@@ -1,0 +1,90 @@
+/*
+ * rsd.h -- thin C ABI between librawspec's C host layer (rawspec_api.c) and
+ * its HIP device layer (rsd_runtime.hip + kernels).  Plain C types only.
+ *
+ * The host layer owns the API semantics of rawspec.h (validation, derived
+ * counts, dump schedule, callbacks, buffer ownership); the device layer owns
+ * device memory, the context's stream and the kernels.  Every call selects
+ * the context's device first, so several contexts (on one or several GPUs)
+ * can live in one process.
+ */
+#ifndef RSD_H_
+#define RSD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSD_MAX_PRODUCTS 4
+
+typedef struct rsd_ctx rsd_ctx;
+
+typedef struct {
+  unsigned Nt;        /* FFT length                                   */
+  unsigned Na;        /* spectra per integration                      */
+  unsigned Nss;       /* spectra per channel per buffer               */
+  unsigned Nds;       /* spectra per dump                             */
+  unsigned Nis;       /* buffers per dump                             */
+  int npolout;        /* 1, 4 or -4                                   */
+} rsd_product;
+
+typedef struct {
+  int gpu_index;
+  unsigned Nc, Np, Ntpb, Nb;
+  unsigned Nbps;      /* 8 or 16 (after complex4 expansion)           */
+  int expand4;        /* host blocks hold complex4 bytes              */
+  unsigned Nant;
+  int conj;           /* input_conjugated                             */
+  int ics;            /* incoherent sum requested                     */
+  const float *weights; /* Nant weights (broadcast already resolved)  */
+  unsigned No;
+  rsd_product prod[RSD_MAX_PRODUCTS];
+} rsd_config;
+
+/* Creates the device context: stream, input buffer, integration buffers,
+ * scratch, twiddle tables.  Returns 0 on success. */
+int  rsd_create(const rsd_config *cfg, rsd_ctx **out);
+void rsd_destroy(rsd_ctx *d);
+
+/* Page-locked host memory (write_combined: host writes only). */
+int  rsd_host_alloc(int gpu_index, void **p, size_t bytes, int write_combined);
+void rsd_host_free(void *p);
+
+/* Synchronous host -> device copies of one host block into device block
+ * slot dblk (reference rawspec_gpu.cu:1441-1469 and :1395-1437). */
+int  rsd_copy_block(rsd_ctx *d, const void *src, unsigned dblk);
+int  rsd_copy_block4(rsd_ctx *d, const void *src, unsigned dblk);
+int  rsd_zero_block(rsd_ctx *d, unsigned dblk);
+
+/* Enqueues the kernels of product i over device input d_in (NULL: the
+ * context's own input buffer).  first != 0 starts a new integration. */
+int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first);
+/* Enqueues the incoherent sum of product i's integration buffer. */
+int  rsd_run_ics(rsd_ctx *d, int i);
+/* Enqueues the device -> host copy of product i's dump (and ICS dump when
+ * h_ics is not NULL). */
+int  rsd_dump_to_host(rsd_ctx *d, int i, float *h_pwr, size_t pwr_bytes,
+                      float *h_ics, size_t ics_bytes);
+/* Enqueues a host function in stream order. */
+int  rsd_host_fn(rsd_ctx *d, void (*fn)(void *), void *arg);
+/* 1: idle, 0: busy, -1: error. */
+int  rsd_query(rsd_ctx *d);
+/* Blocks until the stream drains.  0 on success. */
+int  rsd_sync(rsd_ctx *d);
+
+void  *rsd_input_ptr(rsd_ctx *d);
+size_t rsd_input_bytes(rsd_ctx *d);
+float *rsd_pwr_ptr(rsd_ctx *d, int i);
+float *rsd_ics_ptr(rsd_ctx *d, int i);
+void  *rsd_stream(rsd_ctx *d);
+
+/* Human-readable name of the FFT engine (returned by get_cufft_version). */
+const char *rsd_engine_name(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,434 @@
+// rsd_runtime.hip -- device layer of librawspec: memory, stream, copies,
+// launch planning and kernel dispatch behind the C ABI of rsd.h.
+//
+// HBM layout per context:
+//   d_in    [Nc][Nb*Ntpb][Np][re,im] raw samples (the layout the reference's
+//           2-D block copy builds, rawspec_gpu.cu:1454-1460)
+//   d_acc   per product, [Nds][|P|][Nc][Nt] float: the integration in
+//           progress, already in the host dump layout and FFT-shifted, so a
+//           dump is one contiguous device->host copy
+//   d_ics   per product, [Nds][|P|][Nc/Nant][Nt] float
+//   d_part  partial integrations of groups split over workgroups
+//   d_work  four-step intermediate [Nc][Nss][N1][N2][Np] float2
+//   d_tw    W_4096^m table; per four-step product W_N^i (i < 4096) and
+//           W_N^(4096 h) tables.  All twiddles are computed in FP64 on the
+//           host and rounded once to FP32.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "kernels.h"
+#include "rsd.h"
+
+#define RSD_CHECK(call)                                                           \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      fprintf(stderr, "librawspec: %s failed: %s (%s:%d)\n", #call,               \
+              hipGetErrorString(e_), __FILE__, __LINE__);                         \
+      fflush(stderr);                                                             \
+      return 1;                                                                   \
+    }                                                                             \
+  } while (0)
+
+namespace {
+
+struct ProdPlan {
+  rsd_product p;
+  int Q;
+  int small;               // N <= 4096: fused single kernel
+  // small path
+  unsigned na_buf, Ng, chunk, G, gpw, spg, bpc;
+  int direct;
+  // four-step path
+  unsigned N1, N2;
+  float2 *d_twf, *d_twc;
+  // outputs
+  float *d_acc;
+  size_t acc_len;
+  float *d_ics;
+  size_t ics_len;
+};
+
+bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
+unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
+
+// forward twiddles exp(-2 pi i (m * mul) / n) for m < cnt, FP64 -> FP32
+std::vector<float2> twiddles(unsigned long long n, unsigned long long cnt, unsigned long long mul) {
+  std::vector<float2> t(cnt);
+  for (unsigned long long m = 0; m < cnt; ++m) {
+    const unsigned long long e = (m * mul) % n;
+    const double a = -2.0 * M_PI * (double)e / (double)n;
+    t[m] = make_float2((float)cos(a), (float)sin(a));
+  }
+  return t;
+}
+
+}  // namespace
+
+struct rsd_ctx {
+  int dev;
+  hipStream_t stream;
+  rsd_config cfg;
+  std::vector<float> weights;
+  unsigned long long sample_bytes;     // Np * 2 * Nbps/8
+  unsigned long long chan_block_bytes; // Ntpb * sample_bytes
+  unsigned long long chan_stride;      // Nb * chan_block_bytes
+  uint8_t *d_in;
+  size_t in_bytes;
+  uint8_t *d_stage;                    // one complex4 block
+  float *d_part;
+  size_t part_len;
+  float2 *d_work;
+  size_t work_len;
+  float2 *d_tw;
+  float *d_w;
+  ProdPlan pp[RSD_MAX_PRODUCTS];
+};
+
+static void plan_small(rsd_ctx *d, ProdPlan &pp) {
+  const unsigned N = pp.p.Nt;
+  const unsigned slots = (unsigned)rsk::small_slots((int)N);
+  const unsigned Nc = d->cfg.Nc;
+  pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
+  pp.Ng = pp.p.Nss / pp.na_buf;
+  pp.chunk = pp.na_buf;
+  // Split long groups over workgroups until the launch fills the device
+  // (about 2048 workgroups for 256 CUs), keeping >= one spectrum per slot.
+  const unsigned long long target = 2048;
+  while ((unsigned long long)Nc * pp.Ng * (pp.na_buf / pp.chunk) < target && pp.chunk % 2 == 0 &&
+         pp.chunk / 2 >= slots)
+    pp.chunk /= 2;
+  pp.G = pp.na_buf / pp.chunk;
+  pp.direct = pp.G == 1;
+  // Short groups: several groups per workgroup, a power-of-two number of
+  // slots per group no smaller than the group.
+  pp.gpw = 1;
+  pp.spg = slots;
+  if (pp.G == 1 && pp.chunk < slots) {
+    unsigned spg = 1;
+    while (spg < pp.chunk) spg <<= 1;
+    pp.spg = spg;
+    pp.gpw = slots / spg;
+  }
+  pp.bpc = ((pp.Ng + pp.gpw - 1) / pp.gpw) * pp.G;
+}
+
+static size_t part_need(const rsd_ctx *d, const ProdPlan &pp) {
+  if (!pp.small || pp.direct) return 0;
+  return (size_t)d->cfg.Nc * pp.Ng * pp.G * pp.Q * pp.p.Nt;
+}
+
+static int alloc_zero(void **p, size_t bytes) {
+  if (bytes == 0) { *p = nullptr; return 0; }
+  RSD_CHECK(hipMalloc(p, bytes));
+  RSD_CHECK(hipMemset(*p, 0, bytes));
+  return 0;
+}
+
+extern "C" {
+
+#define RSD_STR1(x) #x
+#define RSD_STR(x) RSD_STR1(x)
+
+const char *rsd_engine_name(void) {
+  return "rawspec-amd gfx950 HIP FFT kernels (HIP " RSD_STR(HIP_VERSION_MAJOR) "." RSD_STR(
+      HIP_VERSION_MINOR) "." RSD_STR(HIP_VERSION_PATCH) ")";
+}
+
+int rsd_host_alloc(int gpu_index, void **p, size_t bytes, int write_combined) {
+  RSD_CHECK(hipSetDevice(gpu_index));
+  RSD_CHECK(hipHostMalloc(p, bytes ? bytes : 1,
+                          write_combined ? hipHostMallocWriteCombined : hipHostMallocDefault));
+  return 0;
+}
+
+void rsd_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
+}
+
+void rsd_destroy(rsd_ctx *d) {
+  if (!d) return;
+  (void)hipSetDevice(d->dev);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  for (int i = 0; i < RSD_MAX_PRODUCTS; ++i) {
+    ProdPlan &pp = d->pp[i];
+    if (pp.d_acc) (void)hipFree(pp.d_acc);
+    if (pp.d_ics) (void)hipFree(pp.d_ics);
+    if (pp.d_twf) (void)hipFree(pp.d_twf);
+    if (pp.d_twc) (void)hipFree(pp.d_twc);
+  }
+  if (d->d_in) (void)hipFree(d->d_in);
+  if (d->d_stage) (void)hipFree(d->d_stage);
+  if (d->d_part) (void)hipFree(d->d_part);
+  if (d->d_work) (void)hipFree(d->d_work);
+  if (d->d_tw) (void)hipFree(d->d_tw);
+  if (d->d_w) (void)hipFree(d->d_w);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
+  d->dev = cfg->gpu_index;
+  d->cfg = *cfg;
+  if (cfg->ics) d->weights.assign(cfg->weights, cfg->weights + cfg->Nant);
+  d->cfg.weights = nullptr;
+  RSD_CHECK(hipSetDevice(d->dev));
+
+  d->sample_bytes = (unsigned long long)cfg->Np * 2 * (cfg->Nbps / 8);
+  d->chan_block_bytes = (unsigned long long)cfg->Ntpb * d->sample_bytes;
+  d->chan_stride = d->chan_block_bytes * cfg->Nb;
+  d->in_bytes = (size_t)(d->chan_stride * cfg->Nc);
+  if (alloc_zero((void **)&d->d_in, d->in_bytes)) return 1;
+  if (cfg->expand4 && alloc_zero((void **)&d->d_stage, (size_t)(d->chan_block_bytes / 2) * cfg->Nc))
+    return 1;
+
+  // master twiddle table W_4096^m
+  {
+    std::vector<float2> t = twiddles(rsk::kTwN, rsk::kTwN, 1);
+    RSD_CHECK(hipMalloc(&d->d_tw, t.size() * sizeof(float2)));
+    RSD_CHECK(hipMemcpy(d->d_tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  if (cfg->ics) {
+    RSD_CHECK(hipMalloc(&d->d_w, cfg->Nant * sizeof(float)));
+    RSD_CHECK(hipMemcpy(d->d_w, d->weights.data(), cfg->Nant * sizeof(float),
+                        hipMemcpyHostToDevice));
+  }
+
+  size_t part_max = 0, work_max = 0;
+  for (unsigned i = 0; i < cfg->No; ++i) {
+    ProdPlan &pp = d->pp[i];
+    pp.p = cfg->prod[i];
+    const unsigned N = pp.p.Nt;
+    if (!is_pow2(N) || N > (1u << 24)) {
+      fprintf(stderr,
+              "librawspec: Nts[%u] = %u is not supported (powers of two up to 2^24 only)\n", i,
+              N);
+      fflush(stderr);
+      return 1;
+    }
+    pp.Q = pp.p.npolout == 1 ? 1 : 4;
+    pp.small = N <= (unsigned)rsk::kTwN;
+    if (pp.small) {
+      plan_small(d, pp);
+    } else {
+      const unsigned L = ilog2u(N);
+      pp.N1 = 1u << ((L + 1) / 2);
+      pp.N2 = 1u << (L / 2);
+      pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
+      pp.Ng = pp.p.Nss / pp.na_buf;
+      std::vector<float2> f = twiddles(N, rsk::kTwN, 1);
+      std::vector<float2> c = twiddles(N, N / rsk::kTwN, rsk::kTwN);
+      RSD_CHECK(hipMalloc(&pp.d_twf, f.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
+      RSD_CHECK(hipMalloc(&pp.d_twc, c.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_twc, c.data(), c.size() * sizeof(float2), hipMemcpyHostToDevice));
+      const size_t wl = (size_t)cfg->Nc * pp.p.Nss * N * cfg->Np;
+      if (wl > work_max) work_max = wl;
+    }
+    const size_t pn = part_need(d, pp);
+    if (pn > part_max) part_max = pn;
+    pp.acc_len = (size_t)pp.Q * pp.p.Nds * cfg->Nc * N;
+    if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
+    if (cfg->ics) {
+      pp.ics_len = pp.acc_len / cfg->Nant;
+      if (alloc_zero((void **)&pp.d_ics, pp.ics_len * sizeof(float))) return 1;
+    }
+  }
+  d->part_len = part_max;
+  if (alloc_zero((void **)&d->d_part, part_max * sizeof(float))) return 1;
+  d->work_len = work_max;
+  if (alloc_zero((void **)&d->d_work, work_max * sizeof(float2))) return 1;
+  RSD_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  return 0;
+}
+
+int rsd_create(const rsd_config *cfg, rsd_ctx **out) {
+  *out = nullptr;
+  rsd_ctx *d = new rsd_ctx();
+  memset(d->pp, 0, sizeof(d->pp));
+  d->stream = nullptr;
+  d->d_in = d->d_stage = nullptr;
+  d->d_part = nullptr;
+  d->d_work = d->d_tw = nullptr;
+  d->d_w = nullptr;
+  if (cfg->ics && cfg->Nc % cfg->Nant != 0) {
+    fprintf(stderr, "librawspec: incoherent sum needs Nc (%u) to be a multiple of Nant (%u)\n",
+            cfg->Nc, cfg->Nant);
+    fflush(stderr);
+    delete d;
+    return 1;
+  }
+  if (create_impl(cfg, d)) {
+    rsd_destroy(d);
+    return 1;
+  }
+  *out = d;
+  return 0;
+}
+
+int rsd_copy_block(rsd_ctx *d, const void *src, unsigned dblk) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipMemcpy2DAsync(d->d_in + dblk * d->chan_block_bytes, d->chan_stride, src,
+                             d->chan_block_bytes, d->chan_block_bytes, d->cfg.Nc,
+                             hipMemcpyHostToDevice, d->stream));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int rsd_copy_block4(rsd_ctx *d, const void *src, unsigned dblk) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  if (!d->d_stage) return 1;
+  const unsigned long long cb4 = d->chan_block_bytes / 2;   // complex4 bytes per channel
+  RSD_CHECK(hipMemcpyAsync(d->d_stage, src, cb4 * d->cfg.Nc, hipMemcpyHostToDevice, d->stream));
+  if (rsk::launch_expand4(d->d_stage, d->d_in, d->cfg.Nc, cb4, d->chan_stride,
+                          dblk * d->chan_block_bytes, d->stream))
+    return 1;
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int rsd_zero_block(rsd_ctx *d, unsigned dblk) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipMemset2DAsync(d->d_in + dblk * d->chan_block_bytes, d->chan_stride, 0,
+                             d->chan_block_bytes, d->cfg.Nc, d->stream));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  ProdPlan &pp = d->pp[i];
+  const rsd_config &c = d->cfg;
+  const uint8_t *in = d_in ? (const uint8_t *)d_in : d->d_in;
+  const float isign = inverse ? -1.0f : 1.0f;
+  // Im cross term: input_conjugated flips it (rawspec_gpu.cu:1120-1123,
+  // 1139-1142); the inverse-via-conjugation trick flips it once more.
+  const float gam = (c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f);
+  const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
+  const float scale = 1.0f / (full * full);
+  if (pp.small) {
+    rsk::SmallArgs a;
+    a.in = in;
+    a.chan_stride = d->chan_stride;
+    a.Nc = c.Nc;
+    a.na_buf = pp.na_buf;
+    a.chunk = pp.chunk;
+    a.G = pp.G;
+    a.Ng = pp.Ng;
+    a.gpw = pp.gpw;
+    a.spg = pp.spg;
+    a.bpc = pp.bpc;
+    a.isign = isign;
+    a.gam = gam;
+    a.scale = scale;
+    a.direct = pp.direct;
+    a.first = first;
+    a.acc = pp.d_acc;
+    a.part = d->d_part;
+    a.tw = d->d_tw;
+    const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
+    if (nb > 0x7fffffffull) return 1;
+    if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
+                          d->stream))
+      return 1;
+    if (!pp.direct) {
+      rsk::ReduceArgs r;
+      r.part = d->d_part;
+      r.acc = pp.d_acc;
+      r.Nc = c.Nc;
+      r.Ng = pp.Ng;
+      r.G = pp.G;
+      r.Q = (unsigned)pp.Q;
+      r.N = pp.p.Nt;
+      r.scale = scale;
+      r.first = first;
+      if (rsk::launch_reduce(r, d->stream)) return 1;
+    }
+    return 0;
+  }
+  rsk::FsArgs a;
+  a.in = in;
+  a.chan_stride = d->chan_stride;
+  a.Nc = c.Nc;
+  a.Nss = pp.p.Nss;
+  a.na_buf = pp.na_buf;
+  a.Ng = pp.Ng;
+  a.N1 = pp.N1;
+  a.N2 = pp.N2;
+  a.isign = isign;
+  a.gam = gam;
+  a.scale = scale;
+  a.first = first;
+  a.work = d->d_work;
+  a.acc = pp.d_acc;
+  a.tw = d->d_tw;
+  a.twf = pp.d_twf;
+  a.twc = pp.d_twc;
+  if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+  if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+  return 0;
+}
+
+int rsd_run_ics(rsd_ctx *d, int i) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  ProdPlan &pp = d->pp[i];
+  if (!pp.d_ics) return 1;
+  rsk::IcsArgs a;
+  a.acc = pp.d_acc;
+  a.ics = pp.d_ics;
+  a.w = d->d_w;
+  a.Nc = d->cfg.Nc;
+  a.Nant = d->cfg.Nant;
+  a.Ncpa = d->cfg.Nc / d->cfg.Nant;
+  a.Q = (unsigned)pp.Q;
+  a.N = pp.p.Nt;
+  a.Nds = pp.p.Nds;
+  return rsk::launch_ics(a, d->stream);
+}
+
+int rsd_dump_to_host(rsd_ctx *d, int i, float *h_pwr, size_t pwr_bytes, float *h_ics,
+                     size_t ics_bytes) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  ProdPlan &pp = d->pp[i];
+  if (pwr_bytes > pp.acc_len * sizeof(float)) return 1;
+  RSD_CHECK(hipMemcpyAsync(h_pwr, pp.d_acc, pwr_bytes, hipMemcpyDeviceToHost, d->stream));
+  if (h_ics && pp.d_ics) {
+    if (ics_bytes > pp.ics_len * sizeof(float)) return 1;
+    RSD_CHECK(hipMemcpyAsync(h_ics, pp.d_ics, ics_bytes, hipMemcpyDeviceToHost, d->stream));
+  }
+  return 0;
+}
+
+int rsd_host_fn(rsd_ctx *d, void (*fn)(void *), void *arg) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipLaunchHostFunc(d->stream, fn, arg));
+  return 0;
+}
+
+int rsd_query(rsd_ctx *d) {
+  if (hipSetDevice(d->dev) != hipSuccess) return -1;
+  hipError_t e = hipStreamQuery(d->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return -1;
+}
+
+int rsd_sync(rsd_ctx *d) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  return 0;
+}
+
+void *rsd_input_ptr(rsd_ctx *d) { return d->d_in; }
+size_t rsd_input_bytes(rsd_ctx *d) { return d->in_bytes; }
+float *rsd_pwr_ptr(rsd_ctx *d, int i) { return d->pp[i].d_acc; }
+float *rsd_ics_ptr(rsd_ctx *d, int i) { return d->pp[i].d_ics; }
+void *rsd_stream(rsd_ctx *d) { return (void *)d->stream; }
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""Shared helpers: run one configuration through librawspec (GPU, via the C
+ABI) and through the FP64 oracle on identical seeded input, and compare.
+
+Error metric (same for every test): for each dump d and coarse channel c,
+let tot[j] be the total-power spectrum of that (d, c) (P for pols=1,
+XX+YY for +4, I for -4).  The error of component q at fine channel j is
+    |gpu[q][j] - ref[q][j]| / max(tot[j], mean_j tot[j])
+i.e. a relative error against the bin's own power, floored at the channel's
+mean power so that Q, U, V and cross terms (which cross zero) and bins of
+near-zero power are judged against the power that carries them.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+PLAIN_TOL = 1e-5      # plain FP32 path vs FP64 oracle (max over all bins)
+
+
+def gen_blocks(Nc, Ntpb, Np, Nbps, nblocks, seed=1, sigma=8.0, tone=None, first_blk=0):
+    """tone = (chan, amp, bin, nfft) or None."""
+    clip = 32767 if Nbps == 16 else 127
+    tc, ta, tb, tn = tone if tone else (-1, 0.0, 0.0, 1.0)
+    return [O.gen_block(Nc, Ntpb, Np, Nbps, seed, sigma, clip, first_blk + b, tc, ta, tb, tn)
+            for b in range(nblocks)]
+
+
+def rel_err(gpu: np.ndarray, ref: np.ndarray, nds: int, q: int, nc: int, nt: int,
+            mode: int) -> np.ndarray:
+    g = np.asarray(gpu, dtype=np.float64).reshape(nds, q, nc, nt)
+    r = np.asarray(ref, dtype=np.float64).reshape(nds, q, nc, nt)
+    if mode == 1:
+        tot = r[:, 0]
+    elif mode == 4:
+        tot = r[:, 0] + r[:, 1]
+    else:
+        tot = r[:, 0]
+    floor = tot.mean(axis=-1, keepdims=True)
+    den = np.maximum(np.maximum(tot, floor), 1e-30)
+    return np.abs(g - r) / den[:, None]
+
+
+def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
+             fft_dir=1, conj=0, Nant=1, ics=False, Aws=None, seed=1, sigma=8.0,
+             tone=None, expand4=False, Nbc=0):
+    """Runs nbuf buffers through GPU and oracle.  Returns a list of
+    (buffer, product, gpu_pwr, ref_pwr, gpu_ics, ref_ics) per dump event and
+    the Spectrometer (closed)."""
+    import rawspec_amd as R
+
+    No = len(Nts)
+    Npolout = list(Npolout) if Npolout is not None else [1] * No
+    sp = R.Spectrometer(Nc=Nc, Ntpb=Ntpb, Nts=Nts, Nas=Nas, Np=Np,
+                        Nbps=4 if expand4 else Nbps, Npolout=Npolout, Nant=Nant,
+                        Nbc=Nbc, Nb=Nb, input_conjugated=conj,
+                        incoherently_sum=1 if ics else 0, Aws=Aws)
+    ctx = sp.ctx
+    cfg = O.make_cfg(No=No, Np=Np, Nant=Nant, Nc=Nc, Ntpb=Ntpb,
+                     Nbps=4 if expand4 else Nbps, Nbc=Nbc, Nb=Nb, Npolout=Npolout,
+                     Nts=Nts, Nas=Nas, input_conjugated=conj,
+                     incoherently_sum=1 if ics else 0,
+                     Naws=len(Aws) if Aws is not None else 0)
+    assert O.derive(cfg) == 0
+    assert cfg.Nb == ctx.Nb and list(cfg.Nds)[:No] == list(ctx.Nds)[:No]
+    st = O.State(cfg, weights=Aws)
+    Nb_ = ctx.Nb
+    dump_bits = 8 if expand4 else Nbps
+    results = []
+    for n in range(nbuf):
+        blocks = gen_blocks(Nc, Ntpb, Np, dump_bits, Nb_, seed=seed, sigma=sigma if not expand4 else 3.0,
+                            tone=tone, first_blk=n * Nb_)
+        if expand4:
+            # complex4 host blocks: clip to the 4-bit range and pack nibbles
+            packed = []
+            for b in blocks:
+                v = np.clip(b.astype(np.int16), -8, 7)
+                re, im = v[0::2], v[1::2]
+                byte = ((re.astype(np.int16) & 0xF) << 4) | (im.astype(np.int16) & 0xF)
+                packed.append(byte.astype(np.uint8))
+                b[:] = v.astype(b.dtype)      # what the expansion must reproduce
+            for bi, pb in enumerate(packed):
+                sp.block(bi)[: pb.size] = pb
+        else:
+            for bi, b in enumerate(blocks):
+                sp.block(bi)[:] = b.view(np.uint8)
+        for i in range(No):
+            sp.pwrbuf(i)[:] = np.frombuffer(b"\x55\x55\x55\x55", dtype=np.float32)[0]
+        ev0 = len(sp.events)
+        rc = sp.copy_and_start(Nb_, expand4=expand4, fft_dir=fft_dir)
+        assert rc == 0
+        assert sp.wait_for_completion() == 0
+        dev = O.device_layout(blocks, Nc, Ntpb, Np)
+        mask, ref, refics = st.process(dev, fft_dir)
+        posts = [p for (p, t) in sp.events[ev0:] if t == 1]
+        assert sorted(posts) == sorted(ref.keys()), (posts, list(ref.keys()))
+        for i in ref:
+            g = sp.pwrbuf(i).copy()
+            gi = sp.icsbuf(i).copy() if ics else None
+            results.append((n, i, g, ref[i], gi, refics.get(i)))
+    events = list(sp.events)
+    sp.close()
+    st.close()
+    return results, cfg, events
+
+
+def check_results(results, cfg, tol=PLAIN_TOL, ics=False):
+    """Asserts every dump is within tol; returns max error per product."""
+    worst = {}
+    for (n, i, g, r, gi, ri) in results:
+        q = abs(cfg.Npolout[i])
+        assert not np.any(g.view(np.uint32) == 0x55555555), f"unwritten output, product {i}"
+        e = rel_err(g, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i])
+        worst[i] = max(worst.get(i, 0.0), float(e.max()))
+        assert e.max() <= tol, f"product {i} buffer {n}: max rel err {e.max():.3e} > {tol}"
+        if ics:
+            ncpa = cfg.Nc // cfg.Nant
+            ei = rel_err(gi, ri, cfg.Nds[i], q, ncpa, cfg.Nts[i], cfg.Npolout[i])
+            assert ei.max() <= tol, f"ICS product {i}: max rel err {ei.max():.3e}"
+    return worst
