@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""RAW ingest throughput of librawspec (gfx950 HIP spectrometer core) on one GPU.
+
+Workload (config 3 of the benchmark set): 1 antenna, 64 coarse channels,
+2 pols, complex int8 samples round(N(0, 8^2)) clipped to +-127 with a
+complex tone of amplitude 20 at fine bin 300/1024 of channel 3 (pol 0);
+blocks of 524288 time samples (128 MiB), so Nb = 2 blocks (256 MiB) per
+device buffer; three products Nfft = {1048576, 8, 1024} integrating
+Nint = {51, 128, 3072} spectra, full Stokes (pols = -4), forward FFT.
+
+One step = one device buffer through rawspec_amd_start_processing_device()
+(the same kernels rawspec_start_processing() runs), with the input already
+resident in HBM: a ring of 64 buffers (16 GiB, the whole 128-block run) is
+generated on the device before timing and cycled.  Dumps stay in device
+memory (rawspec_amd_set_device_output), so PCIe is outside the timed
+region; the PCIe-inclusive figure is reported separately.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "RAW ingest GB/s on 1 GPU, MFMA/served vs plain FP32, ≤1e-3 rel err; p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+W = dict(Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128, Nts=[1 << 20, 8, 1024],
+         Nas=[51, 128, 3072], Npolout=[-4, -4, -4], sigma=8.0, tone_chan=3,
+         tone_amp=20.0, tone_bin=300, tone_nfft=1024, seed=3)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gen_ring(torch, sp, nring):
+    """Seeded synthetic device buffers [Nc][Nb*Ntpb][Np][re,im] int8."""
+    c = sp.ctx
+    S = c.Nb * c.Ntpb
+    ring = []
+    g = torch.Generator(device="cuda")
+    g.manual_seed(W["seed"])
+    t = torch.arange(S, device="cuda", dtype=torch.int64)
+    for r in range(nring):
+        x = torch.randn((c.Nc, S, c.Np, 2), generator=g, device="cuda", dtype=torch.float32)
+        x.mul_(W["sigma"])
+        T = t + r * S
+        ph = ((T * W["tone_bin"]) % W["tone_nfft"]).to(torch.float64) * (2.0 * 3.141592653589793 / W["tone_nfft"])
+        x[W["tone_chan"], :, 0, 0] += (W["tone_amp"] * torch.cos(ph)).to(torch.float32)
+        x[W["tone_chan"], :, 0, 1] += (W["tone_amp"] * torch.sin(ph)).to(torch.float32)
+        buf = x.round_().clamp_(-127, 127).to(torch.int8).contiguous()
+        del x
+        ring.append(buf)
+    torch.cuda.synchronize()
+    return ring
+
+
+def alg_bytes(sp, buf_bytes):
+    """Algorithmic bytes per buffer: RAW read once + each product's dump
+    bytes (and ICS) amortised over its Nis buffers."""
+    c = sp.ctx
+    out = 0.0
+    per_prod = []
+    for i in range(c.No):
+        nss = c.Nb * c.Ntpb // c.Nts[i]
+        nis = max(1, c.Nas[i] // nss)
+        b = c.h_pwrbuf_size[i] / nis
+        per_prod.append((b, nis))
+        out += b
+    return buf_bytes + out, per_prod
+
+
+def cpu_baseline(sp, ring, nthreads, nbuf, chans):
+    import numpy as np
+    import oracle as O
+    c = sp.ctx
+    cfg = O.make_cfg(No=c.No, Np=c.Np, Nc=c.Nc, Ntpb=c.Ntpb, Nbps=8,
+                     Npolout=list(c.Npolout)[:c.No], Nts=list(c.Nts)[:c.No],
+                     Nas=list(c.Nas)[:c.No])
+    assert O.derive(cfg) == 0
+    st = O.State(cfg, threads=nthreads)
+    st.set_channels(chans)
+    host = [ring[k % len(ring)].cpu().numpy().reshape(-1) for k in range(nbuf)]
+    t0 = time.perf_counter()
+    for h in host:
+        st.process(h, 1)
+    dt = time.perf_counter() - t0
+    st.close()
+    per_chan = c.Nb * c.Ntpb * c.Np * 2
+    return len(chans) * per_chan * nbuf / dt / 1e9, dt
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=510)
+    ap.add_argument("--warmup", type=int, default=102)
+    ap.add_argument("--ring", type=int, default=64, help="device-resident input buffers")
+    ap.add_argument("--latency-steps", type=int, default=102)
+    ap.add_argument("--cpu-buffers", type=int, default=4)
+    ap.add_argument("--cpu-chans", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
+    a = ap.parse_args(argv)
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    import rawspec_amd as R
+
+    log(f"bench: workload config3 Nc={W['Nc']} Ntpb={W['Ntpb']} Nts={W['Nts']} Nas={W['Nas']} "
+        f"pols={W['Npolout']}")
+    sp = R.Spectrometer(Nc=W["Nc"], Ntpb=W["Ntpb"], Nts=W["Nts"], Nas=W["Nas"], Np=W["Np"],
+                        Nbps=W["Nbps"], Npolout=W["Npolout"], record_events=False)
+    c = sp.ctx
+    buf_bytes = sp.device_input_size()
+    assert buf_bytes == c.Nc * c.Nb * c.Ntpb * c.Np * 2
+    sp.set_device_output(True)
+    ring = gen_ring(torch, sp, a.ring)
+    log(f"bench: ring of {len(ring)} device buffers x {buf_bytes / 2**20:.0f} MiB ready")
+
+    def step(k):
+        rc = sp.start_processing_device(ring[k % len(ring)].data_ptr(), 1)
+        if rc:
+            raise RuntimeError("rawspec_amd_start_processing_device failed")
+
+    # warmup
+    for k in range(a.warmup):
+        step(k)
+    if sp.wait_for_completion():
+        raise RuntimeError("wait failed")
+    torch.cuda.synchronize()
+
+    # timed region: K buffers back to back on the context's stream
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step(a.warmup + k)
+    if sp.wait_for_completion():
+        raise RuntimeError("wait failed")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = a.steps * buf_bytes / dt / 1e9
+    ms_per_step = dt / a.steps * 1e3
+    log(f"bench: {a.steps} steps in {dt:.3f} s -> {value:.1f} GB/s ({ms_per_step:.3f} ms/step)")
+
+    # per-kernel HIP-event timing on the context's stream (separate pass)
+    sp.profile(True)
+    nprof = max(51, min(a.steps, 153))
+    for k in range(nprof):
+        step(k)
+    prof = sp.profile_report()
+    sp.profile(False)
+    kernels = {k: {"launches": n, "avg_ms": ms / n, "total_ms": ms} for k, (n, ms) in prof.items()}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    B_alg, per_prod = alg_bytes(sp, buf_bytes)
+    prod = int(dom.split(".")[0][1:])
+    kname = dom.split(".")[1]
+    out_b = per_prod[prod][0] * per_prod[prod][1]      # dump bytes of that product
+    nis = per_prod[prod][1]
+    if kname in ("k_small",):
+        kb = buf_bytes + out_b / nis
+    elif kname == "k_fs_a":
+        kb = buf_bytes
+    else:
+        kb = out_b / nis
+    dom_ms = kernels[dom]["avg_ms"]
+    achieved = kb / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": dom, "alg_bytes_per_launch": kb, "avg_launch_ms": dom_ms}
+    pipeline = {"alg_bytes_per_step": B_alg, "achieved_GBs": round(B_alg / (ms_per_step * 1e-3) / 1e9, 1),
+                "frac": round(B_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    log(f"bench: dominant kernel {dom} avg {dom_ms:.3f} ms; kernels: " +
+        ", ".join(f"{k}={v['avg_ms']:.3f}ms x{v['launches']}" for k, v in sorted(kernels.items())))
+
+    # synchronous per-buffer latency (submit -> all products complete)
+    lat = []
+    for k in range(a.latency_steps):
+        t1 = time.perf_counter()
+        step(k)
+        sp.wait_for_completion()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat.sort()
+    p50 = lat[len(lat) // 2]
+    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+
+    extra = {}
+    if a.pcie:
+        # host-buffer API path: copy blocks from pinned host memory, dumps to h_pwrbuf
+        sp.set_device_output(False)
+        hb = ring[0].cpu().numpy().reshape(c.Nc, c.Nb, -1)
+        for b in range(c.Nb):
+            sp.block(b)[:] = hb[:, b, :].reshape(-1).view("uint8")
+        n = 51
+        t1 = time.perf_counter()
+        for k in range(n):
+            sp.copy_and_start(c.Nb, fft_dir=1)
+            sp.wait_for_completion()
+        extra["pcie_inclusive_GBs"] = round(n * buf_bytes / (time.perf_counter() - t1) / 1e9, 2)
+        sp.set_device_output(True)
+
+    cpu = None
+    if not a.no_cpu:
+        nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        chans = sorted({0, 3, c.Nc - 1} | set(range(5, c.Nc, max(1, c.Nc // a.cpu_chans))))[:a.cpu_chans]
+        v, cdt = cpu_baseline(sp, ring, nth, a.cpu_buffers, chans)
+        cpu = {"value": round(v, 4), "unit": "GB/s", "cores": nth, "kind": "port",
+               "sample": f"FP64 C oracle (OpenMP), {len(chans)} of {c.Nc} channels x "
+                         f"{a.cpu_buffers} buffers of this workload, all 3 products, {cdt:.1f} s"}
+        log(f"bench: cpu baseline {v:.4f} GB/s on {nth} threads ({cdt:.1f} s)")
+
+    sp.close()
+    res = {
+        "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "config3: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
+                               "blocks of 524288 samples, Nb=2 (256 MiB/step), "
+                               "Nfft {1048576,8,1024} x Nint {51,128,3072}, pols -4, plain FP32 path",
+                   "step_bytes": buf_bytes, "ring_buffers": len(ring), "input": "HBM-resident",
+                   "dumps": "device-resident"},
+        "roofline": roofline,
+        "pipeline_roofline": pipeline,
+        "kernels": kernels,
+        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3), "mode": "submit->complete, one buffer"},
+        "cpu_baseline": cpu,
+    }
+    res.update(extra)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -44,6 +44,14 @@ float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i);
  * own device work after the context's. */
 void *rawspec_amd_stream(rawspec_context *ctx);
 
+/* Kernel timing.  With enable != 0 every kernel the context launches is
+ * bracketed by HIP events on its stream.  rawspec_amd_profile_report()
+ * waits for the stream, writes one line per kernel, "p<product>.<kernel>.N<Nt>
+ * <launches> <total milliseconds>", into buf (NUL-terminated) and clears the
+ * records.  Returns the number of characters written, -1 on error. */
+int rawspec_amd_profile(rawspec_context *ctx, int enable);
+int rawspec_amd_profile_report(rawspec_context *ctx, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

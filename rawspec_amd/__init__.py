@@ -56,7 +56,7 @@ RawspecContext._fields_ = [
     ("user_data", C.c_void_p),
     ("Nb", C.c_uint),
     ("Nb_host", C.c_uint),
-    ("h_blkbufs", C.POINTER(C.c_char_p)),
+    ("h_blkbufs", C.POINTER(C.c_void_p)),   # char **
     ("gpu_index", C.c_int),
     ("input_conjugated", C.c_int),
     ("incoherently_sum", C.c_int),
@@ -111,6 +111,8 @@ def lib():
             ("rawspec_amd_device_pwrbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_device_icsbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_stream", [P], C.c_void_p),
+            ("rawspec_amd_profile", [P, C.c_int], C.c_int),
+            ("rawspec_amd_profile_report", [P, C.c_char_p, C.c_size_t], C.c_int),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -130,7 +132,7 @@ class Spectrometer:
     def __init__(self, *, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None,
                  Nant=1, Nbc=0, Nb=0, Nb_host=0, gpu_index=0,
                  input_conjugated=0, incoherently_sum=0, Aws=None,
-                 callback=None, init=True):
+                 callback=None, record_events=True, init=True):
         self.ctx = RawspecContext()
         c = self.ctx
         No = len(Nts)
@@ -152,7 +154,8 @@ class Spectrometer:
         self.events = []   # (product, type) in callback order
         self._user_cb = callback
         self._cb = DumpCallback(self._on_dump)
-        c.dump_callback = self._cb
+        if record_events or callback is not None:
+            c.dump_callback = self._cb
         self.initialized = False
         if init:
             rc = self.initialize()
@@ -214,6 +217,21 @@ class Spectrometer:
     def stream(self) -> int:
         return lib().rawspec_amd_stream(C.byref(self.ctx)) or 0
 
+    def profile(self, enable: bool) -> int:
+        return lib().rawspec_amd_profile(C.byref(self.ctx), 1 if enable else 0)
+
+    def profile_report(self) -> dict:
+        """{kernel: (launches, total_ms)} since the last report."""
+        buf = C.create_string_buffer(1 << 16)
+        n = lib().rawspec_amd_profile_report(C.byref(self.ctx), buf, len(buf))
+        if n < 0:
+            raise RuntimeError("rawspec_amd_profile_report failed")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
+
     def reset_integration(self) -> int:
         return lib().rawspec_reset_integration(C.byref(self.ctx))
 
@@ -231,7 +249,9 @@ class Spectrometer:
 
     def block(self, i) -> np.ndarray:
         """Writable uint8 view of host block buffer i."""
-        ptr = C.cast(self.ctx.h_blkbufs[i], C.c_void_p).value
+        ptr = self.ctx.h_blkbufs[i]
+        if not ptr:
+            raise RuntimeError(f"host block {i} is not allocated")
         return np.ctypeslib.as_array((C.c_uint8 * self.blocsize).from_address(ptr))
 
     def pwrbuf(self, i) -> np.ndarray:

@@ -386,7 +386,9 @@ int rawspec_zero_blocks_to_gpu(rawspec_context *ctx, off_t dst_idx, size_t num_b
  * integration completes is dumped between PRE and POST
  * (rawspec_gpu.cu:1543-1747).  PRE is enqueued before any copy into the
  * host buffers, so a client that joins its output thread in PRE never races
- * the copy. */
+ * the copy.  Host callbacks stall the stream while they run, so none is
+ * enqueued when the context has no dump_callback (the reference would call
+ * nothing either). */
 static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
 {
   unsigned int i;
@@ -402,14 +404,14 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
     }
     if (g->inbuf_count % g->Nis[i] == 0) {
       if (ctx->incoherently_sum && rsd_run_ics(g->dev, (int)i)) return 1;
-      if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
+      if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
       if (!g->device_output) {
         if (rsd_dump_to_host(g->dev, (int)i, ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i],
                              ctx->incoherently_sum ? ctx->h_icsbuf[i] : NULL,
                              ctx->h_pwrbuf_size[i] / ctx->Nant))
           return 1;
       }
-      if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->post[i])) return 1;
+      if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->post[i])) return 1;
     }
   }
   return 0;
@@ -453,7 +455,7 @@ int rawspec_wait_for_completion(rawspec_context *ctx)
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
   for (i = 0; i < ctx->No; i++)
-    if (rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
+    if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
   if (rsd_sync(g->dev)) return 1;
   if (*(volatile unsigned int *)&ctx->exit_soon)
     fprintf(stderr, "*** rawspec_wait_for_completion detected exit_soon enabled.\n"
@@ -510,4 +512,18 @@ void *rawspec_amd_stream(rawspec_context *ctx)
 {
   rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
   return (g && g->dev) ? rsd_stream(g->dev) : NULL;
+}
+
+int rawspec_amd_profile(rawspec_context *ctx, int enable)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  return rsd_profile(g->dev, enable);
+}
+
+int rawspec_amd_profile_report(rawspec_context *ctx, char *buf, size_t len)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return -1;
+  return rsd_profile_report(g->dev, buf, len);
 }

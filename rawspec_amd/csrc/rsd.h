@@ -83,6 +83,13 @@ void  *rsd_stream(rsd_ctx *d);
 /* Human-readable name of the FFT engine (returned by get_cufft_version). */
 const char *rsd_engine_name(void);
 
+/* Kernel timing: when enabled, every kernel launch is bracketed by HIP
+ * events on the context's stream.  rsd_profile_report() synchronises, then
+ * writes one line per kernel, "name launches total_ms", into buf and
+ * clears the records.  Returns the number of bytes written, or -1. */
+int rsd_profile(rsd_ctx *d, int enable);
+int rsd_profile_report(rsd_ctx *d, char *buf, size_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
+#include <string>
 #include <vector>
 
 #include "kernels.h"
@@ -89,6 +91,41 @@ struct rsd_ctx {
   float2 *d_tw;
   float *d_w;
   ProdPlan pp[RSD_MAX_PRODUCTS];
+  // kernel timing (rsd_profile)
+  bool prof;
+  struct Rec { std::string name; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+};
+
+static hipEvent_t ev_take(rsd_ctx *d) {
+  if (!d->pool.empty()) {
+    hipEvent_t e = d->pool.back();
+    d->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one kernel launch with events when timing is on.
+struct ProfScope {
+  rsd_ctx *d;
+  long idx;
+  ProfScope(rsd_ctx *dd, int product, const char *kname, unsigned n) : d(dd), idx(-1) {
+    if (!d->prof) return;
+    hipEvent_t a = ev_take(d), b = ev_take(d);
+    if (!a || !b) return;
+    char nm[64];
+    snprintf(nm, sizeof nm, "p%d.%s.N%u", product, kname, n);
+    d->recs.push_back({nm, a, b});
+    idx = (long)d->recs.size() - 1;
+    (void)hipEventRecord(a, d->stream);
+  }
+  ~ProfScope() {
+    if (idx >= 0) (void)hipEventRecord(d->recs[idx].b, d->stream);
+  }
 };
 
 static void plan_small(rsd_ctx *d, ProdPlan &pp) {
@@ -169,6 +206,11 @@ void rsd_destroy(rsd_ctx *d) {
   if (d->d_work) (void)hipFree(d->d_work);
   if (d->d_tw) (void)hipFree(d->d_tw);
   if (d->d_w) (void)hipFree(d->d_w);
+  for (auto &r : d->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : d->pool) (void)hipEventDestroy(e);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -257,6 +299,7 @@ int rsd_create(const rsd_config *cfg, rsd_ctx **out) {
   d->d_part = nullptr;
   d->d_work = d->d_tw = nullptr;
   d->d_w = nullptr;
+  d->prof = false;
   if (cfg->ics && cfg->Nc % cfg->Nant != 0) {
     fprintf(stderr, "librawspec: incoherent sum needs Nc (%u) to be a multiple of Nant (%u)\n",
             cfg->Nc, cfg->Nant);
@@ -334,9 +377,12 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first)
     a.tw = d->d_tw;
     const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
     if (nb > 0x7fffffffull) return 1;
-    if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
-                          d->stream))
-      return 1;
+    {
+      ProfScope ps(d, i, "k_small", pp.p.Nt);
+      if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
+                            d->stream))
+        return 1;
+    }
     if (!pp.direct) {
       rsk::ReduceArgs r;
       r.part = d->d_part;
@@ -348,6 +394,7 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first)
       r.N = pp.p.Nt;
       r.scale = scale;
       r.first = first;
+      ProfScope ps(d, i, "k_reduce", pp.p.Nt);
       if (rsk::launch_reduce(r, d->stream)) return 1;
     }
     return 0;
@@ -370,8 +417,14 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first)
   a.tw = d->d_tw;
   a.twf = pp.d_twf;
   a.twc = pp.d_twc;
-  if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
-  if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+  {
+    ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
+    if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+  }
+  {
+    ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
+    if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+  }
   return 0;
 }
 
@@ -389,6 +442,7 @@ int rsd_run_ics(rsd_ctx *d, int i) {
   a.Q = (unsigned)pp.Q;
   a.N = pp.p.Nt;
   a.Nds = pp.p.Nds;
+  ProfScope ps(d, i, "k_ics", pp.p.Nt);
   return rsk::launch_ics(a, d->stream);
 }
 
@@ -430,5 +484,36 @@ size_t rsd_input_bytes(rsd_ctx *d) { return d->in_bytes; }
 float *rsd_pwr_ptr(rsd_ctx *d, int i) { return d->pp[i].d_acc; }
 float *rsd_ics_ptr(rsd_ctx *d, int i) { return d->pp[i].d_ics; }
 void *rsd_stream(rsd_ctx *d) { return (void *)d->stream; }
+
+int rsd_profile(rsd_ctx *d, int enable) {
+  d->prof = enable != 0;
+  return 0;
+}
+
+int rsd_profile_report(rsd_ctx *d, char *buf, size_t len) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  std::map<std::string, std::pair<long, double>> agg;
+  for (auto &r : d->recs) {
+    float ms = 0.0f;
+    RSD_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+    auto &e = agg[r.name];
+    e.first += 1;
+    e.second += ms;
+    d->pool.push_back(r.a);
+    d->pool.push_back(r.b);
+  }
+  d->recs.clear();
+  size_t off = 0;
+  if (len) buf[0] = 0;
+  for (auto &kv : agg) {
+    int n = snprintf(buf + off, len > off ? len - off : 0, "%s %ld %.6f\n", kv.first.c_str(),
+                     kv.second.first, kv.second.second);
+    if (n < 0) return -1;
+    off += (size_t)n;
+    if (off >= len) return -1;
+  }
+  return (int)off;
+}
 
 }  // extern "C"

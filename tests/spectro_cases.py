@@ -41,6 +41,57 @@ def rel_err(gpu: np.ndarray, ref: np.ndarray, nds: int, q: int, nc: int, nt: int
     return np.abs(g - r) / den[:, None]
 
 
+FP32_FLOOR_NT = 8192   # four-step lengths: judged against the FP32 spur floor
+
+
+class Fp32Ref:
+    """Single-precision reference pipeline (scipy.fft on complex64, float32
+    detection and integration) for one product.  It measures the error any
+    FP32 FFT makes on the same input; for long transforms with a strong tone
+    that floor exceeds 1e-5 (rounding spurs of the tone), so four-step
+    products are held to max(PLAIN_TOL, 2 x this floor)."""
+
+    def __init__(self, cfg, i):
+        self.cfg, self.i = cfg, i
+        self.acc = np.zeros(cfg.pwr_len[i], dtype=np.float32)
+        self.count = 0
+
+    def process(self, dev: np.ndarray, fft_dir: int):
+        import scipy.fft as sf
+        c, i = self.cfg, self.i
+        nt, nss, na = c.Nts[i], c.Nss[i], c.Nas[i]
+        q = abs(c.Npolout[i])
+        nab = min(na, nss)
+        full = 32767.0 if c.Nbps == 16 else 127.0
+        v = dev.reshape(c.Nc, nss, nt, c.Np, 2).astype(np.float32)
+        v = np.maximum(v, -full)
+        x = (v[..., 0] + 1j * v[..., 1]).astype(np.complex64)
+        X = sf.fft(x, axis=2) if fft_dir > 0 else sf.ifft(x, axis=2) * np.float32(nt)
+        X = np.fft.fftshift(X.astype(np.complex64), axes=2)
+        s = np.float32(1.0 / (full * full))
+        g = np.float32(-1.0 if c.input_conjugated else 1.0)
+        X0 = X[..., 0]
+        p0 = (X0.real * X0.real + X0.imag * X0.imag) * s
+        if q == 1:
+            comps = [p0 + ((X[..., 1].real ** 2 + X[..., 1].imag ** 2) * s if c.Np == 2 else 0)]
+        else:
+            X1 = X[..., 1]
+            p1 = (X1.real * X1.real + X1.imag * X1.imag) * s
+            re = (X0.real * X1.real + X0.imag * X1.imag) * s
+            im = g * (X0.imag * X1.real - X0.real * X1.imag) * s
+            comps = [p0, p1, re, im] if c.Npolout[i] == 4 else [p0 + p1, p0 - p1, re, im]
+        P = np.stack(comps, axis=0).astype(np.float32)          # [q][Nc][nss][nt]
+        P = P.reshape(q, c.Nc, nss // nab, nab, nt).sum(axis=3, dtype=np.float32)
+        P = P.transpose(2, 0, 1, 3).reshape(-1)                 # [d][q][Nc][nt]
+        self.acc += P
+        self.count += 1
+        if self.count % c.Nis[i] == 0:
+            out = self.acc.copy()
+            self.acc[:] = 0
+            return out
+        return None
+
+
 def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
              fft_dir=1, conj=0, Nant=1, ics=False, Aws=None, seed=1, sigma=8.0,
              tone=None, expand4=False, Nbc=0):
@@ -64,6 +115,7 @@ def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
     assert O.derive(cfg) == 0
     assert cfg.Nb == ctx.Nb and list(cfg.Nds)[:No] == list(ctx.Nds)[:No]
     st = O.State(cfg, weights=Aws)
+    f32 = {i: Fp32Ref(cfg, i) for i in range(No) if Nts[i] >= FP32_FLOOR_NT}
     Nb_ = ctx.Nb
     dump_bits = 8 if expand4 else Nbps
     results = []
@@ -92,12 +144,13 @@ def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
         assert sp.wait_for_completion() == 0
         dev = O.device_layout(blocks, Nc, Ntpb, Np)
         mask, ref, refics = st.process(dev, fft_dir)
+        f32out = {i: r.process(dev, fft_dir) for i, r in f32.items()}
         posts = [p for (p, t) in sp.events[ev0:] if t == 1]
         assert sorted(posts) == sorted(ref.keys()), (posts, list(ref.keys()))
         for i in ref:
             g = sp.pwrbuf(i).copy()
             gi = sp.icsbuf(i).copy() if ics else None
-            results.append((n, i, g, ref[i], gi, refics.get(i)))
+            results.append((n, i, g, ref[i], gi, refics.get(i), f32out.get(i)))
     events = list(sp.events)
     sp.close()
     st.close()
@@ -107,12 +160,17 @@ def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
 def check_results(results, cfg, tol=PLAIN_TOL, ics=False):
     """Asserts every dump is within tol; returns max error per product."""
     worst = {}
-    for (n, i, g, r, gi, ri) in results:
+    for (n, i, g, r, gi, ri, f32) in results:
         q = abs(cfg.Npolout[i])
         assert not np.any(g.view(np.uint32) == 0x55555555), f"unwritten output, product {i}"
         e = rel_err(g, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i])
         worst[i] = max(worst.get(i, 0.0), float(e.max()))
-        assert e.max() <= tol, f"product {i} buffer {n}: max rel err {e.max():.3e} > {tol}"
+        t = tol
+        if f32 is not None:
+            floor = float(rel_err(f32, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i]).max())
+            t = max(tol, 2.0 * floor)
+            print(f"  product {i}: gpu {e.max():.2e}  fp32-reference floor {floor:.2e}  bound {t:.2e}")
+        assert e.max() <= t, f"product {i} buffer {n}: max rel err {e.max():.3e} > {t}"
         if ics:
             ncpa = cfg.Nc // cfg.Nant
             ei = rel_err(gi, ri, cfg.Nds[i], q, ncpa, cfg.Nts[i], cfg.Npolout[i])
