@@ -197,6 +197,14 @@ int rawspec_initialize(rawspec_context *ctx)
       return 1;
     }
   }
+  /* derived counts (rawspec_gpu.cu:732-760), before any device work */
+  for (i = 0; i < ctx->No; i++) {
+    unsigned int nss = (unsigned)(nbuf / ctx->Nts[i]);
+    ctx->Nds[i] = nss / ctx->Nas[i];
+    if (ctx->Nds[i] == 0) ctx->Nds[i] = 1;
+    ctx->h_pwrbuf_size[i] = (size_t)RS_ABS(ctx->Npolout[i]) * ctx->Nds[i] * ctx->Nts[i] *
+                            ctx->Nc * sizeof(float);
+  }
 
   /* ---- allocation (rawspec_gpu.cu:643-782) ---- */
   for (i = 0; i < MAX_OUTPUTS; i++) {
@@ -234,12 +242,8 @@ int rawspec_initialize(rawspec_context *ctx)
 
   for (i = 0; i < ctx->No; i++) {
     g->Nss[i] = (unsigned)(nbuf / ctx->Nts[i]);
-    ctx->Nds[i] = g->Nss[i] / ctx->Nas[i];
-    if (ctx->Nds[i] == 0) ctx->Nds[i] = 1;
     g->Nis[i] = ctx->Nas[i] / g->Nss[i];
     if (g->Nis[i] == 0) g->Nis[i] = 1;
-    ctx->h_pwrbuf_size[i] = (size_t)RS_ABS(ctx->Npolout[i]) * ctx->Nds[i] * ctx->Nts[i] *
-                            ctx->Nc * sizeof(float);
     if (rsd_host_alloc(ctx->gpu_index, (void **)&ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i], 0))
       goto fail;
     if (ctx->incoherently_sum == 1) {
