@@ -25,9 +25,10 @@ extern "C" {
 void  *rawspec_amd_device_input(rawspec_context *ctx);
 size_t rawspec_amd_device_input_size(rawspec_context *ctx);
 
-/* As rawspec_start_processing(), reading the input from d_in, a device
- * buffer of rawspec_amd_device_input_size() bytes in the same layout on the
- * context's GPU.  d_in must stay unchanged until the work completes. */
+/* As rawspec_start_processing(), reading the input from d_in, a 16-byte
+ * aligned device buffer of rawspec_amd_device_input_size() bytes in the same
+ * layout on the context's GPU.  d_in must stay unchanged until the work
+ * completes. */
 int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in,
                                         int fft_dir);
 

@@ -47,37 +47,38 @@ def _run(cmd):
     return r
 
 
-def _obj(src):
-    return os.path.join(BUILD, os.path.basename(src) + ".o")
-
-
-def compile_one(src):
+def compile_one(src, build_dir=BUILD, defines=()):
     path = os.path.join(CSRC, src)
-    out = _obj(src)
+    out = os.path.join(build_dir, os.path.basename(src) + ".o")
+    dflags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-c", *HIP_FLAGS, "-x", "hip", path, "-o", out]
+        cmd = [HIPCC, "-c", *HIP_FLAGS, *dflags, "-x", "hip", path, "-o", out]
     else:
-        cmd = [CC, "-c", *C_FLAGS, path, "-o", out]
+        cmd = [CC, "-c", *C_FLAGS, *dflags, path, "-o", out]
     _run(cmd)
     return out
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> str:
-    """Compile every source and link librawspec.so; returns its path."""
+def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
+          defines=(), build_dir: str = BUILD) -> str:
+    """Compile every source and link librawspec.so; returns its path.
+
+    defines: extra preprocessor definitions (the kernels' compile-time
+    tunables, e.g. "RSK_E_BIG=8"); out/build_dir redirect the result."""
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(build_dir, exist_ok=True)
     srcs = HIP_SOURCES + C_SOURCES
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: compile_one(s, build_dir, defines), srcs))
+    tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
           "-Wl,--no-undefined", "-Wl,-soname,librawspec.so"])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {out}")
+    return out
 
 
 def exported_symbols(path: str = LIB):

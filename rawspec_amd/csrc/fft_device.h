@@ -96,16 +96,41 @@ struct Dft<2> {
   }
 };
 
+// LDS exchange layout: element `pos` of polarisation p of a team lives at
+// p*lds_len<N>() + lds_pos(pos).  With RSK_LDS_PAD = s > 0 one float2 of
+// padding follows every 2^s elements, which spreads the strided Stockham
+// writes over the banks.
+#ifndef RSK_LDS_PAD
+#define RSK_LDS_PAD 0
+#endif
+__device__ __forceinline__ int lds_pos(int pos) {
+  if constexpr (RSK_LDS_PAD > 0) return pos + (pos >> RSK_LDS_PAD);
+  else return pos;
+}
+template <int N>
+constexpr int lds_len() { return RSK_LDS_PAD > 0 ? N + (N >> RSK_LDS_PAD) : N; }
+
+// Distance in float2 between the LDS regions of consecutive teams of a
+// workgroup; RSK_TEAM_PAD extra float2 keep teams that share a wave off the
+// same banks.
+#ifndef RSK_TEAM_PAD
+#define RSK_TEAM_PAD 8
+#endif
+template <int N, int NP>
+constexpr int team_stride() { return NP * lds_len<N>() + RSK_TEAM_PAD; }
+
 // One Stockham stage of radix R on an N = T*E point team transform, for NP
 // interleaved polarisations.  Butterfly b = j + u*T (u < E/R) reads elements
 // b + r*N/R, which are the thread's own slots u + r*E/R; it writes element r
 // to (b / NS)*NS*R + (b % NS) + r*NS.  The last stage (NS*R == N) writes back
-// into the same register slots, so no exchange follows it.
-template <int N, int E, int NP, int R, int NS, bool LAST>
+// into the same register slots, so no exchange follows it.  tw holds
+// W_TWN^m for m < TWN (TWN a multiple of N).
+template <int N, int E, int NP, int R, int NS, bool LAST, int TWN>
 __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
                                       const float2 *__restrict__ tw) {
   constexpr int T = N / E;
   constexpr int NB = E / R;
+  constexpr int LL = lds_len<N>();
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     const int b = j + u * T;
@@ -113,7 +138,7 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
     float2 w[R];
     if constexpr (NS > 1) {
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[r] = tw[(r * k) * (kTwN / (NS * R))];
+      for (int r = 1; r < R; ++r) w[r] = tw[(r * k) * (TWN / (NS * R))];
     }
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
@@ -131,7 +156,7 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
       } else {
         const int base = (b / NS) * NS * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[p * N + base + r * NS] = a[r];
+        for (int r = 0; r < R; ++r) lds[p * LL + lds_pos(base + r * NS)] = a[r];
       }
     }
   }
@@ -140,49 +165,85 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
-      for (int m = 0; m < E; ++m) v[p][m] = lds[p * N + j + m * T];
+      for (int m = 0; m < E; ++m) v[p][m] = lds[p * LL + lds_pos(j + m * T)];
     __syncthreads();
   }
 }
 
-template <int N, int E, int NP, int NS>
+template <int N, int E, int NP, int NS, int TWN>
 struct Stages {
   __device__ __forceinline__ static void run(float2 (&v)[NP][E], float2 *lds, int j,
                                              const float2 *__restrict__ tw) {
     if constexpr (NS * E == N) {
-      stage<N, E, NP, E, NS, true>(v, lds, j, tw);
+      stage<N, E, NP, E, NS, true, TWN>(v, lds, j, tw);
     } else {
-      stage<N, E, NP, E, NS, false>(v, lds, j, tw);
-      Stages<N, E, NP, NS * E>::run(v, lds, j, tw);
+      stage<N, E, NP, E, NS, false, TWN>(v, lds, j, tw);
+      Stages<N, E, NP, NS * E, TWN>::run(v, lds, j, tw);
     }
   }
 };
 
 // Forward FFT of N points (power of two, E | N, N <= 4096) for NP
-// polarisations held by a team of N/E threads.  lds: NP*N float2 private to
-// the team.  Every thread of the workgroup must call it (barriers inside).
-template <int N, int E, int NP>
+// polarisations held by a team of N/E threads.  lds: NP*lds_len<N>() float2
+// private to the team.  tw: W_TWN^m table (global or LDS).  Every thread of
+// the workgroup must call it (barriers inside).
+template <int N, int E, int NP, int TWN = kTwN>
 __device__ __forceinline__ void team_fft(float2 (&v)[NP][E], float2 *lds, int j,
                                          const float2 *__restrict__ tw) {
   if constexpr (N > 1) {
     constexpr int R0 = 1 << (ilog2(N) % ilog2(E));
     if constexpr (R0 > 1) {
       if constexpr (R0 == N) {
-        stage<N, E, NP, R0, 1, true>(v, lds, j, tw);
+        stage<N, E, NP, R0, 1, true, TWN>(v, lds, j, tw);
       } else {
-        stage<N, E, NP, R0, 1, false>(v, lds, j, tw);
-        Stages<N, E, NP, R0>::run(v, lds, j, tw);
+        stage<N, E, NP, R0, 1, false, TWN>(v, lds, j, tw);
+        Stages<N, E, NP, R0, TWN>::run(v, lds, j, tw);
       }
     } else {
-      Stages<N, E, NP, 1>::run(v, lds, j, tw);
+      Stages<N, E, NP, 1, TWN>::run(v, lds, j, tw);
     }
   }
 }
 
-// Elements per thread for a team transform of N points.
-constexpr int elems_for(int n) {
-  return n <= 16 ? n : (n <= 128 ? 8 : 16);
+// Twiddle source of a team FFT of N points.  With RSK_LDS_TW the workgroup
+// copies W_N^m (m < N) out of the master W_4096 table into LDS once, and the
+// stages read it from there; otherwise they read the master table in global
+// memory.
+#ifndef RSK_LDS_TW
+#define RSK_LDS_TW 1
+#endif
+template <int N>
+struct TwTab {
+  static constexpr int LEN = (RSK_LDS_TW && N > 1) ? N : 1;
+  static constexpr int TWN = RSK_LDS_TW ? (N > 1 ? N : 1) : kTwN;
+};
+template <int N, int WG>
+__device__ __forceinline__ const float2 *tw_setup(float2 *twl, const float2 *tw) {
+  if constexpr (RSK_LDS_TW && N > 1) {
+    for (int m = threadIdx.x; m < N; m += WG) twl[m] = tw[m * (kTwN / N)];
+    __syncthreads();
+    return twl;
+  } else {
+    return tw;
+  }
 }
+
+// Elements per thread for a team transform of N points (compile-time
+// tunables; 16 is the largest radix the butterflies implement).
+#ifndef RSK_E_MID
+#define RSK_E_MID 8
+#endif
+#ifndef RSK_E_BIG
+#define RSK_E_BIG 16
+#endif
+constexpr int elems_for(int n) {
+  return n <= 16 ? n : (n <= 128 ? RSK_E_MID : RSK_E_BIG);
+}
+// Elements per thread of the four-step passes' sub-transforms.
+#ifndef RSK_E_FS
+#define RSK_E_FS 16
+#endif
+constexpr int elems_fs(int n) { return n <= 128 ? 8 : RSK_E_FS; }
 
 // ---- sample unpacking ----------------------------------------------------
 // Raw sample of NP pols with NBPS-bit components -> NP complex floats

@@ -9,83 +9,109 @@
 // (k -> k + N/2 becomes k1 -> k1 + N1/2).  Together they replace the
 // reference's multi-pass cuFFT plan with load/store callbacks
 // (rawspec_gpu.cu:133-267, 1182-1195, 1555-1569) plus its accumulate kernel
-// and shift copies (:297-318, :1647-1697).  Workgroup ids are remapped so each
-// XCD processes a contiguous range of tiles and shares cache lines in its L2.
+// and shift copies (:297-318, :1647-1697).
+//
+// Memory plan: the host launches A and B per chunk of channels sized so the
+// chunk's Y fits in the Infinity Cache, so B reads what A just wrote from
+// cache.  Y is tiled [k2 / TB][n1][k2 % TB][pol] so that a pass-B wave reads
+// 1 KB contiguous per load and a pass-A wave writes 256-byte runs.  For
+// integrations spanning several buffers pass B keeps the running sum in an
+// internal accumulator laid out one contiguous block per workgroup (fully
+// coalesced read-modify-write) and writes the host layout only when the
+// integration completes.  Workgroup ids are remapped so each XCD processes a
+// contiguous range of tiles and shares cache lines in its L2.
 #include "fft_device.h"
 #include "kernels.h"
 
 namespace rsk {
 
-constexpr int kWG = 256;
+// Threads per workgroup for a team transform of n points: 256 or one team.
+constexpr int wg_team(int n) { return n / elems_fs(n) > 256 ? n / elems_fs(n) : 256; }
+constexpr int tile_of(int n) { return wg_team(n) / (n / elems_fs(n)); }
 
 template <int N2, int NBPS, int NP>
-__global__ __launch_bounds__(kWG) void k_fs_a(FsArgs a) {
-  constexpr int E = elems_for(N2);
+__global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
+  constexpr int E = elems_fs(N2);
   constexpr int T = N2 / E;
-  constexpr int TA = kWG / T;          // n1 values per workgroup
+  constexpr int TA = tile_of(N2);      // n1 values per workgroup
   using S = Sample<NBPS, NP>;
   using W = typename S::word;
-  __shared__ float2 lds[TA * NP * N2 > 1 ? TA * NP * N2 : 1];
+  __shared__ float2 lds[TA * team_stride<N2, NP>()];
+  __shared__ float2 twl[TwTab<N2>::LEN];
+  const float2 *twp = tw_setup<N2, wg_team(N2)>(twl, a.tw);
 
   const unsigned N1 = a.N1;
   const unsigned long long N = (unsigned long long)N1 * N2;
   const unsigned tiles = N1 / TA;
-  const unsigned nb = gridDim.x;
-  const unsigned l = xcd_remap(blockIdx.x, nb);
+  const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned tile = l % tiles;
   const unsigned cs = l / tiles;
   const unsigned sp = cs % a.Nss;
-  const unsigned c = cs / a.Nss;
+  const unsigned cl = cs / a.Nss;      // channel within the chunk
   const int tid = threadIdx.x;
   const int sg = tid % TA;             // n1 slot: fastest across lanes
   const int j = tid / TA;
   const unsigned n1 = tile * TA + sg;
 
-  const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)c * a.chan_stride) +
+  const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
                  (unsigned long long)sp * N + n1;
   float2 v[NP][E];
+  W raw[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) raw[m] = src[(unsigned long long)(j + m * T) * N1];
+  // the four-step twiddles W_N^(n1*k2) of this thread's outputs, fetched
+  // while the transform runs
+  float2 tc[E], tf[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const W w = src[(unsigned long long)(j + m * T) * N1];
+    const unsigned e = n1 * (j + m * T);   // < N <= 2^24
+    tc[m] = a.twc[e >> kTwLog];
+    tf[m] = a.twf[e & (kTwN - 1)];
+  }
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
     float2 o[NP];
-    S::cvt(w, o, a.isign);
+    S::cvt(raw[m], o, a.isign);
 #pragma unroll
     for (int p = 0; p < NP; ++p) v[p][m] = o[p];
   }
-  team_fft<N2, E, NP>(v, lds + sg * NP * N2, j, a.tw);
+  team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
 
-  float2 *dst = a.work + ((((unsigned long long)c * a.Nss + sp) * N1 + n1) * N2) * NP;
+  // Y[cl][sp][k2 / TBy][n1][k2 % TBy][pol]
+  float2 *dst = a.work + ((unsigned long long)cl * a.Nss + sp) * N * NP;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const unsigned k2 = j + m * T;
-    const unsigned e = n1 * k2;        // < N <= 2^24
-    const float2 w = cmul(a.twc[e >> kTwLog], a.twf[e & (kTwN - 1)]);
+    const float2 w = cmul(tc[m], tf[m]);
+    const unsigned long long off = ((unsigned long long)(k2 / TBy) * N1 + n1) * TBy + (k2 % TBy);
     if constexpr (NP == 2) {
       const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
-      reinterpret_cast<float4 *>(dst)[k2] = make_float4(y0.x, y0.y, y1.x, y1.y);
+      reinterpret_cast<float4 *>(dst)[off] = make_float4(y0.x, y0.y, y1.x, y1.y);
     } else {
-      dst[k2] = cmul(v[0][m], w);
+      dst[off] = cmul(v[0][m], w);
     }
   }
 }
 
 template <int N1, int NP, int PMODE>
-__global__ __launch_bounds__(kWG) void k_fs_b(FsArgs a) {
-  constexpr int E = elems_for(N1);
+__global__ __launch_bounds__(wg_team(N1)) void k_fs_b(FsArgs a) {
+  constexpr int E = elems_fs(N1);
   constexpr int T = N1 / E;
-  constexpr int TB = kWG / T;          // k2 values per workgroup
+  constexpr int TB = tile_of(N1);      // k2 values per workgroup
   constexpr int Q = nq<PMODE>();
-  __shared__ float2 lds[TB * NP * N1];
+  __shared__ float2 lds[TB * team_stride<N1, NP>()];
+  __shared__ float2 twl[TwTab<N1>::LEN];
+  const float2 *twp = tw_setup<N1, wg_team(N1)>(twl, a.tw);
 
   const unsigned N2 = a.N2;
   const unsigned long long N = (unsigned long long)N1 * N2;
   const unsigned tiles = N2 / TB;
-  const unsigned nb = gridDim.x;
-  const unsigned l = xcd_remap(blockIdx.x, nb);
+  const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned tile = l % tiles;
   const unsigned cg = l / tiles;
   const unsigned g = cg % a.Ng;
-  const unsigned c = cg / a.Ng;
+  const unsigned cl = cg / a.Ng;
+  const unsigned c = a.c0 + cl;
   const int tid = threadIdx.x;
   const int sg = tid % TB;             // k2 slot: fastest across lanes
   const int j = tid / TB;
@@ -98,11 +124,13 @@ __global__ __launch_bounds__(kWG) void k_fs_b(FsArgs a) {
     for (int q = 0; q < Q; ++q) acc[m][q] = 0.0f;
 
   for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
-    const float2 *y = a.work + ((((unsigned long long)c * a.Nss + sp) * N1) * N2 + k2) * NP;
+    // this workgroup's tile: N1 rows of TB k2 values, contiguous
+    const float2 *y = a.work + (((unsigned long long)cl * a.Nss + sp) * N +
+                                (unsigned long long)tile * N1 * TB) * NP;
     float2 v[NP][E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const unsigned long long off = (unsigned long long)(j + m * T) * N2 * NP;
+      const unsigned long long off = ((unsigned long long)(j + m * T) * TB + sg) * NP;
       if constexpr (NP == 2) {
         const float4 t = *reinterpret_cast<const float4 *>(y + off);
         v[0][m] = make_float2(t.x, t.y);
@@ -111,7 +139,7 @@ __global__ __launch_bounds__(kWG) void k_fs_b(FsArgs a) {
         v[0][m] = y[off];
       }
     }
-    team_fft<N1, E, NP>(v, lds + sg * NP * N1, j, a.tw);
+    team_fft<N1, E, NP, TwTab<N1>::TWN>(v, lds + sg * team_stride<N1, NP>(), j, twp);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float2 x[NP];
@@ -121,63 +149,89 @@ __global__ __launch_bounds__(kWG) void k_fs_b(FsArgs a) {
     }
   }
 
-  float *out = a.acc + ((unsigned long long)g * Q * a.Nc + c) * N;
+  // internal accumulator block of this workgroup: [Q][N1][TB]; thread
+  // element (q, m) at ib + (q*N1 + m*T)*TB.  The mode is uniform, so each
+  // branch is straight-line code whose loads are all in flight together.
+  float *ib = a.acc_int + ((((unsigned long long)g * a.Nc + c) * tiles + tile) * Q) * N1 * TB +
+              (unsigned long long)j * TB + sg;
+  float *ob = a.acc + ((unsigned long long)g * Q * a.Nc + c) * N + k2;
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const unsigned k1 = j + m * T;
-    const unsigned long long jo = k2 + (unsigned long long)N2 * ((k1 + N1 / 2) & (N1 - 1));
+  for (int m = 0; m < E; ++m)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      float *o = out + (unsigned long long)q * a.Nc * N + jo;
-      const float val = acc[m][q] * a.scale;
-      *o = a.first ? val : (*o + val);
+    for (int q = 0; q < Q; ++q) acc[m][q] *= a.scale;
+  if (a.mode == FS_MID || a.mode == FS_LAST) {
+    float prev[E][Q];
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) prev[m][q] = ib[((unsigned long long)q * N1 + m * T) * TB];
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) acc[m][q] += prev[m][q];
+  }
+  if (a.mode == FS_FIRST || a.mode == FS_MID) {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) ib[((unsigned long long)q * N1 + m * T) * TB] = acc[m][q];
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const unsigned k1 = j + m * T;
+      const unsigned long long jo = (unsigned long long)N2 * ((k1 + N1 / 2) & (N1 - 1));
+#pragma unroll
+      for (int q = 0; q < Q; ++q) ob[(unsigned long long)q * a.Nc * N + jo] = acc[m][q];
     }
   }
 }
 
-int fs_tile_a(int n2) { return kWG / (n2 / elems_for(n2)); }
-int fs_tile_b(int n1) { return kWG / (n1 / elems_for(n1)); }
+int fs_tile_a(int n2) { return (int)(wg_team(n2) / (n2 / elems_fs(n2))); }
+int fs_tile_b(int n1) { return (int)(wg_team(n1) / (n1 / elems_fs(n1))); }
 
 template <int N2, int NBPS>
-static int go_a(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
-  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, NBPS, 2>), dim3(nb), dim3(kWG), 0, s, a);
-  else hipLaunchKernelGGL((k_fs_a<N2, NBPS, 1>), dim3(nb), dim3(kWG), 0, s, a);
+static int go_a(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
+  constexpr int WGT = wg_team(N2);
+  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a, TBy);
+  else hipLaunchKernelGGL((k_fs_a<N2, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a, TBy);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 template <int NBPS>
-static int go_a_n(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
+static int go_a_n(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
   switch (a.N2) {
-    case 64: return go_a<64, NBPS>(np, a, nb, s);
-    case 128: return go_a<128, NBPS>(np, a, nb, s);
-    case 256: return go_a<256, NBPS>(np, a, nb, s);
-    case 512: return go_a<512, NBPS>(np, a, nb, s);
-    case 1024: return go_a<1024, NBPS>(np, a, nb, s);
-    case 2048: return go_a<2048, NBPS>(np, a, nb, s);
-    case 4096: return go_a<4096, NBPS>(np, a, nb, s);
+    case 64: return go_a<64, NBPS>(np, a, nb, TBy, s);
+    case 128: return go_a<128, NBPS>(np, a, nb, TBy, s);
+    case 256: return go_a<256, NBPS>(np, a, nb, TBy, s);
+    case 512: return go_a<512, NBPS>(np, a, nb, TBy, s);
+    case 1024: return go_a<1024, NBPS>(np, a, nb, TBy, s);
+    case 2048: return go_a<2048, NBPS>(np, a, nb, TBy, s);
+    case 4096: return go_a<4096, NBPS>(np, a, nb, TBy, s);
   }
   return 2;
 }
 
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
-  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.Nc;
-  if (nb == 0 || nb > 0x7fffffffull) return 2;
-  if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);
-  if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, s);
+  const unsigned TBy = (unsigned)fs_tile_b((int)a.N1);
+  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.nch;
+  if (nb == 0 || nb > 0x7fffffffull || a.N2 % TBy != 0) return 2;
+  if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, TBy, s);
+  if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, TBy, s);
   return 2;
 }
 
 template <int N1>
 static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) {
+  constexpr int WGT = wg_team(N1);
   if (np == 1) {
-    hipLaunchKernelGGL((k_fs_b<N1, 1, 1>), dim3(nb), dim3(kWG), 0, s, a);
+    hipLaunchKernelGGL((k_fs_b<N1, 1, 1>), dim3(nb), dim3(WGT), 0, s, a);
   } else if (pmode == 1) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, 1>), dim3(nb), dim3(kWG), 0, s, a);
+    hipLaunchKernelGGL((k_fs_b<N1, 2, 1>), dim3(nb), dim3(WGT), 0, s, a);
   } else if (pmode == 4) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, 4>), dim3(nb), dim3(kWG), 0, s, a);
+    hipLaunchKernelGGL((k_fs_b<N1, 2, 4>), dim3(nb), dim3(WGT), 0, s, a);
   } else if (pmode == -4) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, -4>), dim3(nb), dim3(kWG), 0, s, a);
+    hipLaunchKernelGGL((k_fs_b<N1, 2, -4>), dim3(nb), dim3(WGT), 0, s, a);
   } else {
     return 2;
   }
@@ -186,7 +240,7 @@ static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) 
 
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N2 / fs_tile_b((int)a.N1);
-  const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.Nc;
+  const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.nch;
   if (nb == 0 || nb > 0x7fffffffull) return 2;
   switch (a.N1) {
     case 128: return go_b<128>(np, pmode, a, (unsigned)nb, s);

@@ -18,7 +18,11 @@
 
 namespace rsk {
 
-constexpr int wg_for(int n) { return n <= 16 ? 64 : 256; }
+// Threads per workgroup: one wave for the register-only lengths, otherwise
+// 256 or one team, whichever is larger.
+constexpr int wg_for(int n) {
+  return n <= 16 ? 64 : (n / elems_for(n) > 256 ? n / elems_for(n) : 256);
+}
 
 template <int N, int NBPS, int NP, int PMODE>
 __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
@@ -27,12 +31,13 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
   constexpr int WG = wg_for(N);
   constexpr int SLOTS = WG / T;
   constexpr int Q = nq<PMODE>();
-  constexpr int FFT_F2 = (N > E) ? SLOTS * NP * N : 1;
+  constexpr int FFT_F2 = (N > E) ? SLOTS * team_stride<N, NP>() : 1;
   constexpr int RED_F2 = (SLOTS * Q * N + 1) / 2;
   constexpr int LDS_F2 = FFT_F2 > RED_F2 ? FFT_F2 : RED_F2;
   using S = Sample<NBPS, NP>;
   using W = typename S::word;
   __shared__ float2 lds[LDS_F2];
+  __shared__ float2 twl[TwTab<N>::LEN];
 
   // Workgroup -> (channel, first group, chunk).  gpw groups per workgroup
   // (each with spg = SLOTS/gpw slots) when groups are short; otherwise one
@@ -51,6 +56,7 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
   const unsigned g = g0 + gi;
   const bool gvalid = g < a.Ng;
 
+  const float2 *twp = tw_setup<N, WG>(twl, a.tw);
   const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)c * a.chan_stride);
   const unsigned s0 = (gvalid ? g : 0u) * a.na_buf + chunk * a.chunk;
 
@@ -66,15 +72,25 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
     const bool valid = gvalid && sl < a.chunk;
     const unsigned long long sbase = (unsigned long long)(s0 + (valid ? sl : 0u)) * N;
     float2 v[NP][E];
+    W raw[E];
+    if constexpr (T == 1 && (E * sizeof(W)) % 16 == 0) {
+      // one thread per transform: the spectrum is contiguous, 16-byte loads
+      const uint4 *s4 = reinterpret_cast<const uint4 *>(src + sbase);
+      uint4 *r4 = reinterpret_cast<uint4 *>(raw);
+#pragma unroll
+      for (int m = 0; m < (int)(E * sizeof(W) / 16); ++m) r4[m] = s4[m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) raw[m] = src[sbase + j + m * T];
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const W w = src[sbase + j + m * T];
       float2 o[NP];
-      S::cvt(w, o, a.isign);
+      S::cvt(raw[m], o, a.isign);
 #pragma unroll
       for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
-    team_fft<N, E, NP>(v, lds + slot * NP * N, j, a.tw);
+    team_fft<N, E, NP, TwTab<N>::TWN>(v, lds + slot * team_stride<N, NP>(), j, twp);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -96,29 +112,52 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
     for (int q = 0; q < Q; ++q) red[(slot * Q + q) * N + j + m * T] = acc[m][q];
   __syncthreads();
 
+  // Write-out in batches of OB outputs per thread: sums first, then (when
+  // accumulating) the batch's previous values are loaded together before the
+  // stores, so read-modify-writes do not serialise on memory latency.
+  constexpr int OB = 4;
   const int nout = a.gpw * Q * N;
-  for (int idx = tid; idx < nout; idx += WG) {
-    const int go = idx / (Q * N);
-    const int r2 = idx - go * (Q * N);
-    const int q = r2 / N;
-    const int jo = r2 - q * N;                   // output (shifted) fine channel
-    const int k = (jo + N / 2) & (N - 1);        // FFT bin stored there
-    const unsigned gg = g0 + go;
-    if (gg >= a.Ng) continue;
-    const float *rp = red + ((go * a.spg) * Q + q) * N + k;
-    float sum = rp[0];
-    for (int s2 = 1; s2 < a.spg; ++s2) sum += rp[s2 * Q * N];
-    if (a.direct) {
-      float *o = a.acc + (((unsigned long long)gg * Q + q) * a.Nc + c) * N + jo;
-      const float val = sum * a.scale;
-      *o = a.first ? val : (*o + val);
-    } else {
-      a.part[((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q + q) * N + jo] = sum;
+  float *base = a.direct ? a.acc : a.part;
+  const bool accum = a.direct && !a.first;
+  for (int i0 = tid; i0 < nout; i0 += OB * WG) {
+    float val[OB];
+    unsigned long long dst[OB];
+#pragma unroll
+    for (int u = 0; u < OB; ++u) {
+      const int idx = i0 + u * WG;
+      const int go = idx / (Q * N);
+      const int r2 = idx - go * (Q * N);
+      const int q = r2 / N;
+      const int jo = r2 - q * N;                   // output (shifted) fine channel
+      const int k = (jo + N / 2) & (N - 1);        // FFT bin stored there
+      const unsigned gg = g0 + go;
+      dst[u] = ~0ull;
+      val[u] = 0.0f;
+      if (idx < nout && gg < a.Ng) {
+        const float *rp = red + ((go * a.spg) * Q + q) * N + k;
+        float sum = rp[0];
+        for (int s2 = 1; s2 < a.spg; ++s2) sum += rp[s2 * Q * N];
+        if (a.direct) {
+          dst[u] = (((unsigned long long)gg * Q + q) * a.Nc + c) * N + jo;
+          val[u] = sum * a.scale;
+        } else {
+          dst[u] = ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q + q) * N + jo;
+          val[u] = sum;
+        }
+      }
     }
+    if (accum) {
+#pragma unroll
+      for (int u = 0; u < OB; ++u)
+        if (dst[u] != ~0ull) val[u] += base[dst[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < OB; ++u)
+      if (dst[u] != ~0ull) base[dst[u]] = val[u];
   }
 }
 
-int small_wg(int n) { return n <= 16 ? 64 : 256; }
+int small_wg(int n) { return wg_for(n); }
 int small_slots(int n) { return small_wg(n) / (n / elems_for(n) > 0 ? n / elems_for(n) : 1); }
 
 template <int N, int NBPS, int NP, int PMODE>

@@ -35,18 +35,31 @@ struct SmallArgs {
   const float2 *tw;               // W_4096^m, m < 4096
 };
 
-// Four-step path (N = N1*N2 > 4096).  Pass A: length-N2 FFTs over stride N1
-// input, twiddle W_N^(n1*k2), store Y[n1][k2][pol].  Pass B: length-N1 FFTs
-// per k2, detect, integrate the group's spectra, write FFT-shifted output.
+// Four-step path (N = N1*N2 > 4096), launched per chunk of channels
+// [c0, c0 + nch) so the intermediate stays cache-resident.
+// Pass A: length-N2 FFTs over stride-N1 input, twiddle W_N^(n1*k2), store Y
+// tiled for pass B: Y[cl][s][k2 / TB][n1][k2 % TB][pol].
+// Pass B: length-N1 FFTs per k2, detection, integration of the group's
+// spectra; output per `mode`:
+//   FS_HOST   write the FFT-shifted host layout (integration complete)
+//   FS_FIRST  write the internal accumulator (first buffer of Nis > 1)
+//   FS_MID    add into the internal accumulator
+//   FS_LAST   internal + this buffer -> host layout
+// Internal accumulator: [Ng][Nc][N2/TB][Q][N1][TB] (one contiguous block
+// per workgroup).
+enum { FS_HOST = 0, FS_FIRST = 1, FS_MID = 2, FS_LAST = 3 };
+
 struct FsArgs {
   const uint8_t *in;
   unsigned long long chan_stride;
   unsigned Nc, Nss, na_buf, Ng;
+  unsigned c0, nch;               // channel chunk
   unsigned N1, N2;
   float isign, gam, scale;
-  int first;
-  float2 *work;                   // [Nc][Nss][N1][N2][Np]
-  float *acc;
+  int mode;
+  float2 *work;                   // Y for the chunk
+  float *acc;                     // host layout [Nds][Q][Nc][N]
+  float *acc_int;                 // internal accumulator (Nis > 1)
   const float2 *tw;               // W_4096^m
   const float2 *twf;              // W_N^i, i < 4096
   const float2 *twc;              // W_N^(4096 h), h < N/4096

@@ -401,7 +401,8 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
   g->inbuf_count++;
   for (i = 0; i < ctx->No; i++) {
     int first = ((g->inbuf_count - 1) % g->Nis[i]) == 0;
-    if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first)) {
+    int last = (g->inbuf_count % g->Nis[i]) == 0;
+    if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first, last)) {
       fprintf(stderr, "librawspec: processing output product %u failed\n", i);
       fflush(stderr);
       return 1;
@@ -483,8 +484,8 @@ size_t rawspec_amd_device_input_size(rawspec_context *ctx)
 
 int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in, int fft_dir)
 {
-  if (!d_in) {
-    rs_err("rawspec_amd_start_processing_device: NULL input");
+  if (!d_in || ((size_t)d_in & 15) != 0) {
+    rs_err("rawspec_amd_start_processing_device: input must be a 16-byte aligned device pointer");
     return 1;
   }
   return rs_start(ctx, d_in, fft_dir);

@@ -59,8 +59,9 @@ int  rsd_copy_block4(rsd_ctx *d, const void *src, unsigned dblk);
 int  rsd_zero_block(rsd_ctx *d, unsigned dblk);
 
 /* Enqueues the kernels of product i over device input d_in (NULL: the
- * context's own input buffer).  first != 0 starts a new integration. */
-int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first);
+ * context's own input buffer).  first != 0 starts a new integration;
+ * last != 0 completes one (the dump follows). */
+int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last);
 /* Enqueues the incoherent sum of product i's integration buffer. */
 int  rsd_run_ics(rsd_ctx *d, int i);
 /* Enqueues the device -> host copy of product i's dump (and ICS dump when

@@ -49,13 +49,22 @@ struct ProdPlan {
   int direct;
   // four-step path
   unsigned N1, N2;
+  unsigned fs_chunk;       // channels per pass A/B launch pair
   float2 *d_twf, *d_twc;
+  float *d_acc_int;        // running sum across buffers (Nis > 1)
   // outputs
   float *d_acc;
   size_t acc_len;
   float *d_ics;
   size_t ics_len;
 };
+
+// Four-step intermediate budget per pass A/B launch pair: well inside the
+// MI355X's 256 MB Infinity Cache so pass B's reads hit it.
+#ifndef RSD_FS_CACHE_MB
+#define RSD_FS_CACHE_MB 96
+#endif
+constexpr size_t kFsCacheBytes = (size_t)RSD_FS_CACHE_MB << 20;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -143,15 +152,20 @@ static void plan_small(rsd_ctx *d, ProdPlan &pp) {
     pp.chunk /= 2;
   pp.G = pp.na_buf / pp.chunk;
   pp.direct = pp.G == 1;
-  // Short groups: several groups per workgroup, a power-of-two number of
-  // slots per group no smaller than the group.
+  // Slots per group: a power of two, no larger than the group and at most
+  // kMaxSpg, so each slot integrates several spectra in registers and the
+  // final cross-slot sum stays short; the remaining slots take further
+  // groups (gpw groups per workgroup, only when the group is not split).
+  const unsigned kMaxSpg = 16;
   pp.gpw = 1;
   pp.spg = slots;
-  if (pp.G == 1 && pp.chunk < slots) {
+  if (pp.G == 1) {
     unsigned spg = 1;
-    while (spg < pp.chunk) spg <<= 1;
-    pp.spg = spg;
-    pp.gpw = slots / spg;
+    while (spg < pp.chunk && spg < kMaxSpg) spg <<= 1;
+    if (spg < slots) {
+      pp.spg = spg;
+      pp.gpw = slots / spg;
+    }
   }
   pp.bpc = ((pp.Ng + pp.gpw - 1) / pp.gpw) * pp.G;
 }
@@ -199,6 +213,7 @@ void rsd_destroy(rsd_ctx *d) {
     if (pp.d_ics) (void)hipFree(pp.d_ics);
     if (pp.d_twf) (void)hipFree(pp.d_twf);
     if (pp.d_twc) (void)hipFree(pp.d_twc);
+    if (pp.d_acc_int) (void)hipFree(pp.d_acc_int);
   }
   if (d->d_in) (void)hipFree(d->d_in);
   if (d->d_stage) (void)hipFree(d->d_stage);
@@ -270,13 +285,21 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
       RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
       RSD_CHECK(hipMalloc(&pp.d_twc, c.size() * sizeof(float2)));
       RSD_CHECK(hipMemcpy(pp.d_twc, c.data(), c.size() * sizeof(float2), hipMemcpyHostToDevice));
-      const size_t wl = (size_t)cfg->Nc * pp.p.Nss * N * cfg->Np;
+      // channels per A/B launch pair: the chunk's intermediate stays within
+      // kFsCacheBytes so pass B reads it back from the Infinity Cache
+      const size_t per_chan = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
+      size_t ch = kFsCacheBytes / per_chan;
+      pp.fs_chunk = (unsigned)(ch < 1 ? 1 : (ch > cfg->Nc ? cfg->Nc : ch));
+      const size_t wl = (size_t)pp.fs_chunk * pp.p.Nss * N * cfg->Np;
       if (wl > work_max) work_max = wl;
     }
     const size_t pn = part_need(d, pp);
     if (pn > part_max) part_max = pn;
     pp.acc_len = (size_t)pp.Q * pp.p.Nds * cfg->Nc * N;
     if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
+    if (!pp.small && pp.p.Nis > 1 &&
+        alloc_zero((void **)&pp.d_acc_int, pp.acc_len * sizeof(float)))
+      return 1;
     if (cfg->ics) {
       pp.ics_len = pp.acc_len / cfg->Nant;
       if (alloc_zero((void **)&pp.d_ics, pp.ics_len * sizeof(float))) return 1;
@@ -344,7 +367,7 @@ int rsd_zero_block(rsd_ctx *d, unsigned dblk) {
   return 0;
 }
 
-int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first) {
+int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last) {
   RSD_CHECK(hipSetDevice(d->dev));
   ProdPlan &pp = d->pp[i];
   const rsd_config &c = d->cfg;
@@ -411,19 +434,28 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first)
   a.isign = isign;
   a.gam = gam;
   a.scale = scale;
-  a.first = first;
+  if (pp.p.Nis == 1) a.mode = rsk::FS_HOST;
+  else if (first) a.mode = rsk::FS_FIRST;
+  else if (last) a.mode = rsk::FS_LAST;
+  else a.mode = rsk::FS_MID;
   a.work = d->d_work;
   a.acc = pp.d_acc;
+  a.acc_int = pp.d_acc_int;
   a.tw = d->d_tw;
   a.twf = pp.d_twf;
   a.twc = pp.d_twc;
-  {
-    ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
-    if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
-  }
-  {
-    ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
-    if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+  // pass A and pass B alternate per channel chunk (cache-resident Y)
+  for (unsigned c0 = 0; c0 < c.Nc; c0 += pp.fs_chunk) {
+    a.c0 = c0;
+    a.nch = c.Nc - c0 < pp.fs_chunk ? c.Nc - c0 : pp.fs_chunk;
+    {
+      ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
+      if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+    }
+    {
+      ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
+      if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+    }
   }
   return 0;
 }
