@@ -105,8 +105,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=102)
     ap.add_argument("--ring", type=int, default=64, help="device-resident input buffers")
     ap.add_argument("--latency-steps", type=int, default=102)
-    ap.add_argument("--cpu-buffers", type=int, default=4)
-    ap.add_argument("--cpu-chans", type=int, default=16)
+    ap.add_argument("--cpu-buffers", type=int, default=8)
+    ap.add_argument("--cpu-chans", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
     a = ap.parse_args(argv)
@@ -168,12 +168,16 @@ def main(argv=None):
     kname = dom.split(".")[1]
     out_b = per_prod[prod][0] * per_prod[prod][1]      # dump bytes of that product
     nis = per_prod[prod][1]
+    # algorithmic bytes of the kernel per step: RAW bytes it must read and
+    # dump bytes it must write (amortised over Nis); split evenly over its
+    # launches per step (the four-step passes run once per channel chunk)
     if kname in ("k_small",):
-        kb = buf_bytes + out_b / nis
+        kb_step = buf_bytes + out_b / nis
     elif kname == "k_fs_a":
-        kb = buf_bytes
+        kb_step = buf_bytes
     else:
-        kb = out_b / nis
+        kb_step = out_b / nis
+    kb = kb_step / (kernels[dom]["launches"] / nprof)
     dom_ms = kernels[dom]["avg_ms"]
     achieved = kb / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -101,7 +101,7 @@ struct Dft<2> {
 // padding follows every 2^s elements, which spreads the strided Stockham
 // writes over the banks.
 #ifndef RSK_LDS_PAD
-#define RSK_LDS_PAD 0
+#define RSK_LDS_PAD 5
 #endif
 __device__ __forceinline__ int lds_pos(int pos) {
   if constexpr (RSK_LDS_PAD > 0) return pos + (pos >> RSK_LDS_PAD);
@@ -116,8 +116,13 @@ constexpr int lds_len() { return RSK_LDS_PAD > 0 ? N + (N >> RSK_LDS_PAD) : N; }
 #ifndef RSK_TEAM_PAD
 #define RSK_TEAM_PAD 8
 #endif
+// RSK_SEQ_POL = 1: the polarisations go through the LDS exchange one after
+// the other (half the LDS per team, twice the barriers).
+#ifndef RSK_SEQ_POL
+#define RSK_SEQ_POL 1
+#endif
 template <int N, int NP>
-constexpr int team_stride() { return NP * lds_len<N>() + RSK_TEAM_PAD; }
+constexpr int team_stride() { return (RSK_SEQ_POL ? 1 : NP) * lds_len<N>() + RSK_TEAM_PAD; }
 
 // One Stockham stage of radix R on an N = T*E point team transform, for NP
 // interleaved polarisations.  Butterfly b = j + u*T (u < E/R) reads elements
@@ -131,6 +136,33 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
   constexpr int T = N / E;
   constexpr int NB = E / R;
   constexpr int LL = lds_len<N>();
+  if constexpr (RSK_SEQ_POL && !LAST && NP > 1) {
+    // one polarisation at a time through a single LDS region
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        const int b = j + u * T;
+        const int k = b & (NS - 1);
+        float2 a[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) a[r] = v[p][u + r * NB];
+        if constexpr (NS > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tw[(r * k) * (TWN / (NS * R))]);
+        }
+        Dft<R>::run(a);
+        const int base = (b / NS) * NS * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_pos(base + r * NS)] = a[r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < E; ++m) v[p][m] = lds[lds_pos(j + m * T)];
+      __syncthreads();
+    }
+    return;
+  }
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     const int b = j + u * T;

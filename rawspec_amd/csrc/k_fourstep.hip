@@ -25,8 +25,25 @@
 
 namespace rsk {
 
-// Threads per workgroup for a team transform of n points: 256 or one team.
-constexpr int wg_team(int n) { return n / elems_fs(n) > 256 ? n / elems_fs(n) : 256; }
+// Threads per workgroup for a team transform of n points: RSK_FS_WG or one
+// team, whichever is larger.  More teams per workgroup means longer
+// contiguous runs in pass A's input gather and pass B's output.
+#ifndef RSK_FS_WG
+#define RSK_FS_WG 256
+#endif
+constexpr int kLdsBytes = 160 * 1024;
+template <int NP>
+constexpr int wg_team_np(int n) {
+  // as many teams as RSK_FS_WG asks for, as far as their LDS regions fit
+  const int t = n / elems_fs(n);
+  const int per_team = (RSK_SEQ_POL ? 1 : NP) * (RSK_LDS_PAD > 0 ? n + (n >> RSK_LDS_PAD) : n) +
+                       RSK_TEAM_PAD;
+  const int tw = (RSK_LDS_TW && n > 1) ? n : 1;
+  int teams = RSK_FS_WG / t > 0 ? RSK_FS_WG / t : 1;
+  while (teams > 1 && (teams * per_team + tw) * 8 > kLdsBytes) teams /= 2;
+  return teams * t;
+}
+constexpr int wg_team(int n) { return wg_team_np<2>(n); }
 constexpr int tile_of(int n) { return wg_team(n) / (n / elems_fs(n)); }
 
 template <int N2, int NBPS, int NP>
