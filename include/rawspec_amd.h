@@ -19,6 +19,12 @@
 extern "C" {
 #endif
 
+/* Validates and derives exactly as rawspec_initialize() does (updating the
+ * same fields of ctx) but allocates nothing; reports the device bytes and the
+ * pinned host bytes rawspec_initialize() would allocate.  For admission
+ * control by HBM budget.  Returns 0 when the configuration is valid. */
+int rawspec_amd_plan(rawspec_context *ctx, size_t *device_bytes, size_t *host_bytes);
+
 /* Device pointer and size of the context's input buffer, laid out
  * [Nc][Nb*Ntpb][Np][re,im] with Nbps-bit components (the layout
  * rawspec_copy_blocks_to_gpu builds).  NULL / 0 before initialisation. */

@@ -104,6 +104,7 @@ def lib():
             ("rawspec_reset_integration", [P], C.c_int),
             ("rawspec_check_for_completion", [P], C.c_uint),
             ("rawspec_wait_for_completion", [P], C.c_int),
+            ("rawspec_amd_plan", [P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
             ("rawspec_amd_device_input", [P], C.c_void_p),
             ("rawspec_amd_device_input_size", [P], C.c_size_t),
             ("rawspec_amd_start_processing_device", [P, C.c_void_p, C.c_int], C.c_int),

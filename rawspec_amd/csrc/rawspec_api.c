@@ -71,16 +71,16 @@ static unsigned long rs_mod(off_t x, unsigned long n)
   return (unsigned long)(r < 0 ? r + (long long)n : r);
 }
 
-int rawspec_initialize(rawspec_context *ctx)
+/* Validation and derived counts, same rules and order as
+ * rawspec_gpu.cu:471-641 and :732-760.  Updates the ctx fields the reference
+ * updates (Npolout, Nbps, Ntmax, Nb, Nb_host, Nant, Nbc, Nds,
+ * h_pwrbuf_size).  Returns 0 when the configuration is valid. */
+static int rs_validate(rawspec_context *ctx, int *expand4_out, unsigned long long *nbuf_out)
 {
   unsigned int i;
   int expand4;
-  rs_gpu_ctx *g;
-  rsd_config cfg;
-  float *weights = NULL;
   unsigned long long nbuf;
 
-  /* ---- validation, same rules and order as rawspec_gpu.cu:471-641 ---- */
   if (ctx->No == 0 || ctx->No > MAX_OUTPUTS) {
     fprintf(stderr, "output products must be in range [1..%d], not %d\n", MAX_OUTPUTS, ctx->No);
     fflush(stderr);
@@ -205,6 +205,50 @@ int rawspec_initialize(rawspec_context *ctx)
     ctx->h_pwrbuf_size[i] = (size_t)RS_ABS(ctx->Npolout[i]) * ctx->Nds[i] * ctx->Nts[i] *
                             ctx->Nc * sizeof(float);
   }
+  *expand4_out = expand4;
+  *nbuf_out = nbuf;
+  return 0;
+}
+
+/* The device layer's view of a validated context (weights excluded). */
+static void rs_fill_config(const rawspec_context *ctx, int expand4, unsigned long long nbuf,
+                           rsd_config *cfg)
+{
+  unsigned int i;
+  memset(cfg, 0, sizeof(*cfg));
+  cfg->gpu_index = ctx->gpu_index;
+  cfg->Nc = ctx->Nc;
+  cfg->Np = ctx->Np;
+  cfg->Ntpb = ctx->Ntpb;
+  cfg->Nb = ctx->Nb;
+  cfg->Nbps = ctx->Nbps;
+  cfg->expand4 = expand4;
+  cfg->Nant = ctx->Nant;
+  cfg->conj = ctx->input_conjugated ? 1 : 0;
+  cfg->ics = ctx->incoherently_sum ? 1 : 0;
+  cfg->No = ctx->No;
+  for (i = 0; i < ctx->No; i++) {
+    unsigned int nss = (unsigned)(nbuf / ctx->Nts[i]);
+    unsigned int nis = ctx->Nas[i] / nss;
+    cfg->prod[i].Nt = ctx->Nts[i];
+    cfg->prod[i].Na = ctx->Nas[i];
+    cfg->prod[i].Nss = nss;
+    cfg->prod[i].Nds = ctx->Nds[i];
+    cfg->prod[i].Nis = nis ? nis : 1;
+    cfg->prod[i].npolout = ctx->Npolout[i];
+  }
+}
+
+int rawspec_initialize(rawspec_context *ctx)
+{
+  unsigned int i;
+  int expand4;
+  rs_gpu_ctx *g;
+  rsd_config cfg;
+  float *weights = NULL;
+  unsigned long long nbuf;
+
+  if (rs_validate(ctx, &expand4, &nbuf)) return 1;
 
   /* ---- allocation (rawspec_gpu.cu:643-782) ---- */
   for (i = 0; i < MAX_OUTPUTS; i++) {
@@ -253,32 +297,13 @@ int rawspec_initialize(rawspec_context *ctx)
     }
   }
 
-  memset(&cfg, 0, sizeof(cfg));
-  cfg.gpu_index = ctx->gpu_index;
-  cfg.Nc = ctx->Nc;
-  cfg.Np = ctx->Np;
-  cfg.Ntpb = ctx->Ntpb;
-  cfg.Nb = ctx->Nb;
-  cfg.Nbps = ctx->Nbps;
-  cfg.expand4 = expand4;
-  cfg.Nant = ctx->Nant;
-  cfg.conj = ctx->input_conjugated ? 1 : 0;
-  cfg.ics = ctx->incoherently_sum ? 1 : 0;
-  cfg.No = ctx->No;
+  rs_fill_config(ctx, expand4, nbuf, &cfg);
   if (cfg.ics) {
     /* one broadcast weight or one per antenna (rawspec_gpu.cu:1040-1055) */
     weights = (float *)malloc(ctx->Nant * sizeof(float));
     if (!weights) goto fail;
     for (i = 0; i < ctx->Nant; i++) weights[i] = ctx->Naws == 1 ? ctx->Aws[0] : ctx->Aws[i];
     cfg.weights = weights;
-  }
-  for (i = 0; i < ctx->No; i++) {
-    cfg.prod[i].Nt = ctx->Nts[i];
-    cfg.prod[i].Na = ctx->Nas[i];
-    cfg.prod[i].Nss = g->Nss[i];
-    cfg.prod[i].Nds = ctx->Nds[i];
-    cfg.prod[i].Nis = g->Nis[i];
-    cfg.prod[i].npolout = ctx->Npolout[i];
   }
   if (rsd_create(&cfg, &g->dev)) goto fail;
   free(weights);
@@ -531,4 +556,26 @@ int rawspec_amd_profile_report(rawspec_context *ctx, char *buf, size_t len)
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return -1;
   return rsd_profile_report(g->dev, buf, len);
+}
+
+int rawspec_amd_plan(rawspec_context *ctx, size_t *device_bytes, size_t *host_bytes)
+{
+  int expand4;
+  unsigned long long nbuf;
+  unsigned int i;
+  rsd_config cfg;
+  size_t d, h = 0;
+  if (rs_validate(ctx, &expand4, &nbuf)) return 1;
+  rs_fill_config(ctx, expand4, nbuf, &cfg);
+  d = rsd_plan_bytes(&cfg);
+  if (d == 0) return 1;
+  if (!ctx->h_blkbufs)
+    h += (size_t)ctx->Nb_host * ctx->Ntpb * ctx->Np * ctx->Nc * 2 * (ctx->Nbps / 8);
+  for (i = 0; i < ctx->No; i++) {
+    h += ctx->h_pwrbuf_size[i];
+    if (ctx->incoherently_sum == 1) h += ctx->h_pwrbuf_size[i] / ctx->Nant;
+  }
+  if (device_bytes) *device_bytes = d;
+  if (host_bytes) *host_bytes = h;
+  return 0;
 }

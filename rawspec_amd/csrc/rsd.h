@@ -46,6 +46,9 @@ typedef struct {
 /* Creates the device context: stream, input buffer, integration buffers,
  * scratch, twiddle tables.  Returns 0 on success. */
 int  rsd_create(const rsd_config *cfg, rsd_ctx **out);
+/* Device bytes rsd_create would allocate for cfg (0 if cfg is unsupported);
+ * touches no device. */
+size_t rsd_plan_bytes(const rsd_config *cfg);
 void rsd_destroy(rsd_ctx *d);
 
 /* Page-locked host memory (write_combined: host writes only). */

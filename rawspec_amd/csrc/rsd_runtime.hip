@@ -230,33 +230,21 @@ void rsd_destroy(rsd_ctx *d) {
   delete d;
 }
 
-static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
+// Sizes and launch plans for a configuration, without touching the device.
+static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
   d->dev = cfg->gpu_index;
   d->cfg = *cfg;
-  if (cfg->ics) d->weights.assign(cfg->weights, cfg->weights + cfg->Nant);
   d->cfg.weights = nullptr;
-  RSD_CHECK(hipSetDevice(d->dev));
-
+  if (cfg->ics && (cfg->Nant == 0 || cfg->Nc % cfg->Nant != 0)) {
+    fprintf(stderr, "librawspec: incoherent sum needs Nc (%u) to be a multiple of Nant (%u)\n",
+            cfg->Nc, cfg->Nant);
+    fflush(stderr);
+    return 1;
+  }
   d->sample_bytes = (unsigned long long)cfg->Np * 2 * (cfg->Nbps / 8);
   d->chan_block_bytes = (unsigned long long)cfg->Ntpb * d->sample_bytes;
   d->chan_stride = d->chan_block_bytes * cfg->Nb;
   d->in_bytes = (size_t)(d->chan_stride * cfg->Nc);
-  if (alloc_zero((void **)&d->d_in, d->in_bytes)) return 1;
-  if (cfg->expand4 && alloc_zero((void **)&d->d_stage, (size_t)(d->chan_block_bytes / 2) * cfg->Nc))
-    return 1;
-
-  // master twiddle table W_4096^m
-  {
-    std::vector<float2> t = twiddles(rsk::kTwN, rsk::kTwN, 1);
-    RSD_CHECK(hipMalloc(&d->d_tw, t.size() * sizeof(float2)));
-    RSD_CHECK(hipMemcpy(d->d_tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
-  }
-  if (cfg->ics) {
-    RSD_CHECK(hipMalloc(&d->d_w, cfg->Nant * sizeof(float)));
-    RSD_CHECK(hipMemcpy(d->d_w, d->weights.data(), cfg->Nant * sizeof(float),
-                        hipMemcpyHostToDevice));
-  }
-
   size_t part_max = 0, work_max = 0;
   for (unsigned i = 0; i < cfg->No; ++i) {
     ProdPlan &pp = d->pp[i];
@@ -279,12 +267,6 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
       pp.N2 = 1u << (L / 2);
       pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
       pp.Ng = pp.p.Nss / pp.na_buf;
-      std::vector<float2> f = twiddles(N, rsk::kTwN, 1);
-      std::vector<float2> c = twiddles(N, N / rsk::kTwN, rsk::kTwN);
-      RSD_CHECK(hipMalloc(&pp.d_twf, f.size() * sizeof(float2)));
-      RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
-      RSD_CHECK(hipMalloc(&pp.d_twc, c.size() * sizeof(float2)));
-      RSD_CHECK(hipMemcpy(pp.d_twc, c.data(), c.size() * sizeof(float2), hipMemcpyHostToDevice));
       // channels per A/B launch pair: the chunk's intermediate stays within
       // kFsCacheBytes so pass B reads it back from the Infinity Cache
       const size_t per_chan = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
@@ -296,25 +278,68 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
     const size_t pn = part_need(d, pp);
     if (pn > part_max) part_max = pn;
     pp.acc_len = (size_t)pp.Q * pp.p.Nds * cfg->Nc * N;
-    if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
-    if (!pp.small && pp.p.Nis > 1 &&
-        alloc_zero((void **)&pp.d_acc_int, pp.acc_len * sizeof(float)))
-      return 1;
-    if (cfg->ics) {
-      pp.ics_len = pp.acc_len / cfg->Nant;
-      if (alloc_zero((void **)&pp.d_ics, pp.ics_len * sizeof(float))) return 1;
-    }
+    pp.ics_len = cfg->ics ? pp.acc_len / cfg->Nant : 0;
   }
   d->part_len = part_max;
-  if (alloc_zero((void **)&d->d_part, part_max * sizeof(float))) return 1;
   d->work_len = work_max;
-  if (alloc_zero((void **)&d->d_work, work_max * sizeof(float2))) return 1;
+  return 0;
+}
+
+// Device bytes the plan allocates.
+static size_t plan_bytes(const rsd_ctx *d) {
+  size_t b = d->in_bytes + (d->cfg.expand4 ? (size_t)(d->chan_block_bytes / 2) * d->cfg.Nc : 0);
+  b += (size_t)rsk::kTwN * sizeof(float2) + (d->cfg.ics ? d->cfg.Nant * sizeof(float) : 0);
+  for (unsigned i = 0; i < d->cfg.No; ++i) {
+    const ProdPlan &pp = d->pp[i];
+    b += pp.acc_len * sizeof(float) + pp.ics_len * sizeof(float);
+    if (!pp.small) {
+      b += (rsk::kTwN + pp.p.Nt / rsk::kTwN) * sizeof(float2);
+      if (pp.p.Nis > 1) b += pp.acc_len * sizeof(float);
+    }
+  }
+  return b + d->part_len * sizeof(float) + d->work_len * sizeof(float2);
+}
+
+static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
+  if (plan_all(cfg, d)) return 1;
+  if (cfg->ics) d->weights.assign(cfg->weights, cfg->weights + cfg->Nant);
+  RSD_CHECK(hipSetDevice(d->dev));
+  if (alloc_zero((void **)&d->d_in, d->in_bytes)) return 1;
+  if (cfg->expand4 && alloc_zero((void **)&d->d_stage, (size_t)(d->chan_block_bytes / 2) * cfg->Nc))
+    return 1;
+  // master twiddle table W_4096^m
+  {
+    std::vector<float2> t = twiddles(rsk::kTwN, rsk::kTwN, 1);
+    RSD_CHECK(hipMalloc(&d->d_tw, t.size() * sizeof(float2)));
+    RSD_CHECK(hipMemcpy(d->d_tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  if (cfg->ics) {
+    RSD_CHECK(hipMalloc(&d->d_w, cfg->Nant * sizeof(float)));
+    RSD_CHECK(hipMemcpy(d->d_w, d->weights.data(), cfg->Nant * sizeof(float),
+                        hipMemcpyHostToDevice));
+  }
+  for (unsigned i = 0; i < cfg->No; ++i) {
+    ProdPlan &pp = d->pp[i];
+    const unsigned N = pp.p.Nt;
+    if (!pp.small) {
+      std::vector<float2> f = twiddles(N, rsk::kTwN, 1);
+      std::vector<float2> c = twiddles(N, N / rsk::kTwN, rsk::kTwN);
+      RSD_CHECK(hipMalloc(&pp.d_twf, f.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
+      RSD_CHECK(hipMalloc(&pp.d_twc, c.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_twc, c.data(), c.size() * sizeof(float2), hipMemcpyHostToDevice));
+      if (pp.p.Nis > 1 && alloc_zero((void **)&pp.d_acc_int, pp.acc_len * sizeof(float))) return 1;
+    }
+    if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
+    if (cfg->ics && alloc_zero((void **)&pp.d_ics, pp.ics_len * sizeof(float))) return 1;
+  }
+  if (alloc_zero((void **)&d->d_part, d->part_len * sizeof(float))) return 1;
+  if (alloc_zero((void **)&d->d_work, d->work_len * sizeof(float2))) return 1;
   RSD_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
   return 0;
 }
 
-int rsd_create(const rsd_config *cfg, rsd_ctx **out) {
-  *out = nullptr;
+static rsd_ctx *new_ctx() {
   rsd_ctx *d = new rsd_ctx();
   memset(d->pp, 0, sizeof(d->pp));
   d->stream = nullptr;
@@ -323,19 +348,25 @@ int rsd_create(const rsd_config *cfg, rsd_ctx **out) {
   d->d_work = d->d_tw = nullptr;
   d->d_w = nullptr;
   d->prof = false;
-  if (cfg->ics && cfg->Nc % cfg->Nant != 0) {
-    fprintf(stderr, "librawspec: incoherent sum needs Nc (%u) to be a multiple of Nant (%u)\n",
-            cfg->Nc, cfg->Nant);
-    fflush(stderr);
-    delete d;
-    return 1;
-  }
+  return d;
+}
+
+int rsd_create(const rsd_config *cfg, rsd_ctx **out) {
+  *out = nullptr;
+  rsd_ctx *d = new_ctx();
   if (create_impl(cfg, d)) {
     rsd_destroy(d);
     return 1;
   }
   *out = d;
   return 0;
+}
+
+size_t rsd_plan_bytes(const rsd_config *cfg) {
+  rsd_ctx *d = new_ctx();
+  size_t b = plan_all(cfg, d) ? 0 : plan_bytes(d);
+  delete d;
+  return b;
 }
 
 int rsd_copy_block(rsd_ctx *d, const void *src, unsigned dblk) {

@@ -126,3 +126,62 @@ def test_library_validation_matches_oracle(rlib, capfd):
             assert c.h_pwrbuf_size[i] == 4 * cfg.pwr_len[i]
     capfd.readouterr()
     assert nvalid > 50 and ninvalid > 50
+
+
+def test_plan_validates_like_initialize_and_sizes_buffers(rlib, capfd):
+    """rawspec_amd_plan() applies rawspec_initialize()'s validation and
+    derivation without allocating (it runs here, with no GPU) and reports at
+    least the input buffer plus one integration buffer per product on the
+    device, and exactly the pinned host buffers rawspec_initialize() would
+    allocate."""
+    rng = random.Random(77)
+    nvalid = ninvalid = 0
+    for _ in range(1500):
+        kw = _random_cfg(rng)
+        cfg = O.make_cfg(**kw)
+        orc = O.derive(cfg)
+        c = rlib.RawspecContext()
+        for k in ("No", "Np", "Nc", "Ntpb", "Nbps", "Nbc", "Nb", "Nant"):
+            setattr(c, k, kw[k])
+        for i in range(kw["No"]):
+            c.Nts[i], c.Nas[i], c.Npolout[i] = kw["Nts"][i], kw["Nas"][i], kw["Npolout"][i]
+        dev, host = C.c_size_t(0), C.c_size_t(0)
+        rc = rlib.lib().rawspec_amd_plan(C.byref(c), C.byref(dev), C.byref(host))
+        pow2 = all(n & (n - 1) == 0 for n in kw["Nts"][:kw["No"]])
+        if orc != 0 or not pow2:
+            ninvalid += 1
+            assert rc == 1, kw
+            continue
+        nvalid += 1
+        assert rc == 0, kw
+        assert not c.gpu_ctx
+        assert c.Nds[0] == cfg.Nds[0] and c.Nb == cfg.Nb
+        pwr = sum(c.h_pwrbuf_size[i] for i in range(kw["No"]))
+        blocks = cfg.Nb_host * kw["Ntpb"] * kw["Np"] * kw["Nc"] * 2 * (cfg.Nbps // 8)
+        assert host.value == pwr + blocks, kw
+        in_bytes = cfg.Nb * kw["Ntpb"] * kw["Np"] * kw["Nc"] * 2 * (cfg.Nbps // 8)
+        assert dev.value >= in_bytes + pwr, kw
+    capfd.readouterr()
+    assert nvalid > 50 and ninvalid > 50
+
+
+def test_plan_bytes_for_the_64_channel_three_product_workload(rlib):
+    """64 channels x 2 pols x 8 bits, 2 blocks of 2^19 samples; products
+    2^20 x 51, 8 x 128, 1024 x 3072, full Stokes.  The device plan holds the
+    256 MiB input, the integration buffers (1 GiB for 2^20 and its 1 GiB
+    inter-buffer accumulator since it integrates over 51 buffers, 8 MiB for
+    Nfft 8 with 1024 spectra per dump, 1 MiB for 1024), the 96 MiB four-step
+    intermediate for 6 channels, and twiddle tables; the rest is the small
+    path's partial sums, bounded by one more integration buffer set."""
+    c = rlib.RawspecContext()
+    c.No, c.Np, c.Nc, c.Ntpb, c.Nbps, c.Nb = 3, 2, 64, 1 << 19, 8, 2
+    for i, (nt, na) in enumerate([(1 << 20, 51), (8, 128), (1024, 3072)]):
+        c.Nts[i], c.Nas[i], c.Npolout[i] = nt, na, -4
+    dev, host = C.c_size_t(0), C.c_size_t(0)
+    assert rlib.lib().rawspec_amd_plan(C.byref(c), C.byref(dev), C.byref(host)) == 0
+    MiB = 1 << 20
+    fixed = 256 * MiB + 2 * 1024 * MiB + 8 * MiB + 1 * MiB + 96 * MiB
+    tw = (4096 + (4096 + 256)) * 8
+    assert fixed + tw <= dev.value <= fixed + tw + 64 * MiB, dev.value
+    # pinned host: three dump buffers (blocks are allocated too: h_blkbufs NULL)
+    assert host.value == 1024 * MiB + 8 * MiB + 1 * MiB + 2 * 256 * MiB // 2
