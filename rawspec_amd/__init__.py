@@ -85,6 +85,16 @@ def lib():
             raise RuntimeError(
                 f"{_LIB_PATH} is missing: build it with `python -m rawspec_amd.build` "
                 "(there is no fallback implementation)")
+        # One HIP runtime per process.  torch's libraries resolve the
+        # unversioned libamdhip64.so from torch's own directory, while this
+        # library binds libamdhip64.so.7 by soname.  Loaded after torch, this
+        # library shares torch's runtime; loaded first, it would make torch
+        # load a second runtime, which then finds no GPU.  So torch, when it
+        # is installed, is loaded first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(_LIB_PATH)
         P = C.POINTER(RawspecContext)
         L.get_librawspec_version.restype = C.c_char_p
