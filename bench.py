@@ -1,11 +1,11 @@
 #!/usr/bin/env python3
 """RAW ingest throughput of librawspec (gfx950 HIP spectrometer core) on one GPU.
 
-Workload (config 3 of the benchmark set): 1 antenna, 64 coarse channels,
-2 pols, complex int8 samples round(N(0, 8^2)) clipped to +-127 with a
-complex tone of amplitude 20 at fine bin 300/1024 of channel 3 (pol 0);
-blocks of 524288 time samples (128 MiB), so Nb = 2 blocks (256 MiB) per
-device buffer; three products Nfft = {1048576, 8, 1024} integrating
+Headline workload (config 3 of the benchmark set): 1 antenna, 64 coarse
+channels, 2 pols, complex int8 samples round(N(0, 8^2)) clipped to +-127
+with a complex tone of amplitude 20 at fine bin 300/1024 of channel 3
+(pol 0); blocks of 524288 time samples (128 MiB), so Nb = 2 blocks (256 MiB)
+per device buffer; three products Nfft = {1048576, 8, 1024} integrating
 Nint = {51, 128, 3072} spectra, full Stokes (pols = -4), forward FFT.
 
 One step = one device buffer through rawspec_amd_start_processing_device()
@@ -13,7 +13,14 @@ One step = one device buffer through rawspec_amd_start_processing_device()
 resident in HBM: a ring of 64 buffers (16 GiB, the whole 128-block run) is
 generated on the device before timing and cycled.  Dumps stay in device
 memory (rawspec_amd_set_device_output), so PCIe is outside the timed
-region; the PCIe-inclusive figure is reported separately.
+region; the PCIe-inclusive figure is reported separately (--pcie).
+
+Also measured, reported under "configs": the other single-stream workloads
+(config 1: 8 ch, 4 MiB buffers, Nfft 1024; config 2: config 3's geometry
+with pols 1 and pols 4; config 4: 16 antennas x 16 channels of complex
+int16, four products with the incoherent sum) and, under "serving", the
+serving loop (32 Poisson-arriving streams of random shapes, concurrent vs
+one at a time).
 
 Prints one JSON line.
 """
@@ -21,7 +28,9 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
+import random
 import sys
 import time
 
@@ -32,19 +41,50 @@ if ROOT not in sys.path:
 METRIC = "RAW ingest GB/s on 1 GPU, MFMA/served vs plain FP32, ≤1e-3 rel err; p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-W = dict(Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128, Nts=[1 << 20, 8, 1024],
-         Nas=[51, 128, 3072], Npolout=[-4, -4, -4], sigma=8.0, tone_chan=3,
-         tone_amp=20.0, tone_bin=300, tone_nfft=1024, seed=3)
+C3 = dict(name="config3", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128, Nts=[1 << 20, 8, 1024],
+          Nas=[51, 128, 3072], Npolout=[-4, -4, -4], sigma=8.0, tone=(3, 20.0, 300, 1024),
+          seed=3, Nant=1)
+OTHER = [
+    dict(name="config1", Nc=8, Np=2, Nbps=8, Ntpb=131072, nblocks=4, Nts=[1024], Nas=[16],
+         Npolout=[1], sigma=8.0, tone=(3, 20.0, 300, 1024), seed=1, Nant=1),
+    dict(name="config2_pols1", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
+         Nts=[1 << 20, 8, 1024], Nas=[51, 128, 3072], Npolout=[1, 1, 1], sigma=8.0, tone=None,
+         seed=2, Nant=1),
+    dict(name="config2_pols4", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
+         Nts=[1 << 20, 8, 1024], Nas=[51, 128, 3072], Npolout=[4, 4, 4], sigma=8.0, tone=None,
+         seed=2, Nant=1),
+    dict(name="config4", Nc=256, Np=2, Nbps=16, Ntpb=262144, nblocks=64,
+         Nts=[64, 4096, 131072, 8], Nas=[2048, 32, 4, 16384], Npolout=[1, 1, 1, 1],
+         sigma=1000.0, tone=None, seed=4, Nant=16),
+]
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def gen_ring(torch, sp, nring):
-    """Seeded synthetic device buffers [Nc][Nb*Ntpb][Np][re,im] int8."""
+def weights(W):
+    if W["Nant"] <= 1:
+        return None
+    rng = random.Random(W["seed"])
+    return [rng.uniform(0.5, 1.5) for _ in range(W["Nant"])]
+
+
+def make_sp(R, W):
+    aws = weights(W)
+    kw = dict(incoherently_sum=1, Aws=aws) if aws else {}
+    return R.Spectrometer(Nc=W["Nc"], Ntpb=W["Ntpb"], Nts=W["Nts"], Nas=W["Nas"], Np=W["Np"],
+                          Nbps=W["Nbps"], Npolout=W["Npolout"], Nant=W["Nant"],
+                          record_events=False, **kw)
+
+
+def gen_ring(torch, sp, W, nring):
+    """Seeded synthetic device buffers [Nc][Nb*Ntpb][Np][re,im] (int8 or
+    int16): round(N(0, sigma^2)) plus the optional tone (pol 0, phase
+    continuous in the sample index), clipped to the signed range."""
     c = sp.ctx
     S = c.Nb * c.Ntpb
+    lim = 32767 if W["Nbps"] == 16 else 127
     ring = []
     g = torch.Generator(device="cuda")
     g.manual_seed(W["seed"])
@@ -52,13 +92,14 @@ def gen_ring(torch, sp, nring):
     for r in range(nring):
         x = torch.randn((c.Nc, S, c.Np, 2), generator=g, device="cuda", dtype=torch.float32)
         x.mul_(W["sigma"])
-        T = t + r * S
-        ph = ((T * W["tone_bin"]) % W["tone_nfft"]).to(torch.float64) * (2.0 * 3.141592653589793 / W["tone_nfft"])
-        x[W["tone_chan"], :, 0, 0] += (W["tone_amp"] * torch.cos(ph)).to(torch.float32)
-        x[W["tone_chan"], :, 0, 1] += (W["tone_amp"] * torch.sin(ph)).to(torch.float32)
-        buf = x.round_().clamp_(-127, 127).to(torch.int8).contiguous()
+        if W["tone"]:
+            ch, amp, k, n = W["tone"]
+            ph = (((t + r * S) * k) % n).to(torch.float64) * (2.0 * math.pi / n)
+            x[ch, :, 0, 0] += (amp * torch.cos(ph)).to(torch.float32)
+            x[ch, :, 0, 1] += (amp * torch.sin(ph)).to(torch.float32)
+        x = x.round_().clamp_(-lim, lim)
+        ring.append(x.to(torch.int16 if W["Nbps"] == 16 else torch.int8).contiguous())
         del x
-        ring.append(buf)
     torch.cuda.synchronize()
     return ring
 
@@ -72,14 +113,32 @@ def alg_bytes(sp, buf_bytes):
     for i in range(c.No):
         nss = c.Nb * c.Ntpb // c.Nts[i]
         nis = max(1, c.Nas[i] // nss)
-        b = c.h_pwrbuf_size[i] / nis
-        per_prod.append((b, nis))
+        b = c.h_pwrbuf_size[i] * (1 + (1.0 / c.Nant if c.incoherently_sum else 0.0)) / nis
+        per_prod.append((b * nis, nis))
         out += b
     return buf_bytes + out, per_prod
 
 
+def timed(torch, sp, ring, steps, warmup):
+    def step(k):
+        if sp.start_processing_device(ring[k % len(ring)].data_ptr(), 1):
+            raise RuntimeError("rawspec_amd_start_processing_device failed")
+
+    for k in range(warmup):
+        step(k)
+    if sp.wait_for_completion():
+        raise RuntimeError("wait failed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k)
+    if sp.wait_for_completion():
+        raise RuntimeError("wait failed")
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, step
+
+
 def cpu_baseline(sp, ring, nthreads, nbuf, chans):
-    import numpy as np
     import oracle as O
     c = sp.ctx
     cfg = O.make_cfg(No=c.No, Np=c.Np, Nc=c.Nc, Ntpb=c.Ntpb, Nbps=8,
@@ -98,6 +157,42 @@ def cpu_baseline(sp, ring, nthreads, nbuf, chans):
     return len(chans) * per_chan * nbuf / dt / 1e9, dt
 
 
+def other_config(torch, R, W, nring=8):
+    sp = make_sp(R, W)
+    sp.set_device_output(True)
+    buf_bytes = sp.device_input_size()
+    ring = gen_ring(torch, sp, W, nring)
+    nbuf_run = max(1, W["nblocks"] // sp.ctx.Nb)          # the whole run of blocks
+    steps = max(nbuf_run, 32)
+    dt, _ = timed(torch, sp, ring, steps, warmup=max(4, steps // 8))
+    B_alg, _ = alg_bytes(sp, buf_bytes)
+    ms = dt / steps * 1e3
+    out = {"GBps": round(steps * buf_bytes / dt / 1e9, 2), "ms_per_step": round(ms, 4),
+           "step_bytes": buf_bytes, "steps": steps,
+           "pipeline_frac_of_hbm": round(B_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    sp.close()
+    del ring
+    torch.cuda.empty_cache()
+    return out
+
+
+def serving(torch):
+    import rawspec_amd.serve as S
+    reqs = S.make_requests(n=32, rate=4.0, seed=5)
+    out = {"streams": len(reqs), "arrivals": "Poisson, mean 4/s, seed 5",
+           "ingest_bytes": sum(r.nbuf * S.plan(r.shape).buf_bytes for r in reqs)}
+    for name, kw, scale in [("concurrent", {}, 1.0), ("sequential", {"max_active": 1}, 1.0),
+                            ("concurrent_all_present", {}, 0.0),
+                            ("sequential_all_present", {"max_active": 1}, 0.0)]:
+        st = S.Server(**kw).run(reqs, arrival_scale=scale)
+        st.pop("results")
+        out[name] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()
+                     if k not in ("streams", "ingest_bytes")}
+        log(f"bench: serving {name}: {st['GBps']:.1f} GB/s, p50 {st['p50_ms']:.2f} ms, "
+            f"p99 {st['p99_ms']:.2f} ms, admission wait p99 {st['admit_wait_p99_ms']:.1f} ms")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +203,8 @@ def main(argv=None):
     ap.add_argument("--cpu-buffers", type=int, default=8)
     ap.add_argument("--cpu-chans", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true", help="headline workload only")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
     a = ap.parse_args(argv)
 
@@ -119,37 +216,18 @@ def main(argv=None):
     import torch
     import rawspec_amd as R
 
+    W = C3
     log(f"bench: workload config3 Nc={W['Nc']} Ntpb={W['Ntpb']} Nts={W['Nts']} Nas={W['Nas']} "
         f"pols={W['Npolout']}")
-    sp = R.Spectrometer(Nc=W["Nc"], Ntpb=W["Ntpb"], Nts=W["Nts"], Nas=W["Nas"], Np=W["Np"],
-                        Nbps=W["Nbps"], Npolout=W["Npolout"], record_events=False)
+    sp = make_sp(R, W)
     c = sp.ctx
     buf_bytes = sp.device_input_size()
     assert buf_bytes == c.Nc * c.Nb * c.Ntpb * c.Np * 2
     sp.set_device_output(True)
-    ring = gen_ring(torch, sp, a.ring)
+    ring = gen_ring(torch, sp, W, a.ring)
     log(f"bench: ring of {len(ring)} device buffers x {buf_bytes / 2**20:.0f} MiB ready")
 
-    def step(k):
-        rc = sp.start_processing_device(ring[k % len(ring)].data_ptr(), 1)
-        if rc:
-            raise RuntimeError("rawspec_amd_start_processing_device failed")
-
-    # warmup
-    for k in range(a.warmup):
-        step(k)
-    if sp.wait_for_completion():
-        raise RuntimeError("wait failed")
-    torch.cuda.synchronize()
-
-    # timed region: K buffers back to back on the context's stream
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        step(a.warmup + k)
-    if sp.wait_for_completion():
-        raise RuntimeError("wait failed")
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    dt, step = timed(torch, sp, ring, a.steps, a.warmup)
     value = a.steps * buf_bytes / dt / 1e9
     ms_per_step = dt / a.steps * 1e3
     log(f"bench: {a.steps} steps in {dt:.3f} s -> {value:.1f} GB/s ({ms_per_step:.3f} ms/step)")
@@ -166,8 +244,7 @@ def main(argv=None):
     B_alg, per_prod = alg_bytes(sp, buf_bytes)
     prod = int(dom.split(".")[0][1:])
     kname = dom.split(".")[1]
-    out_b = per_prod[prod][0] * per_prod[prod][1]      # dump bytes of that product
-    nis = per_prod[prod][1]
+    out_b, nis = per_prod[prod]                     # dump bytes of that product, buffers per dump
     # algorithmic bytes of the kernel per step: RAW bytes it must read and
     # dump bytes it must write (amortised over Nis); split evenly over its
     # launches per step (the four-step passes run once per channel chunk)
@@ -225,6 +302,16 @@ def main(argv=None):
         log(f"bench: cpu baseline {v:.4f} GB/s on {nth} threads ({cdt:.1f} s)")
 
     sp.close()
+    del ring
+    torch.cuda.empty_cache()
+
+    configs = {}
+    if not a.no_others:
+        for Wo in OTHER:
+            configs[Wo["name"]] = other_config(torch, R, Wo)
+            log(f"bench: {Wo['name']}: {configs[Wo['name']]}")
+    serve = None if a.no_serve else serving(torch)
+
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
@@ -233,13 +320,15 @@ def main(argv=None):
         "config": {"workload": "config3: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
                                "blocks of 524288 samples, Nb=2 (256 MiB/step), "
                                "Nfft {1048576,8,1024} x Nint {51,128,3072}, pols -4, plain FP32 path",
-                   "step_bytes": buf_bytes, "ring_buffers": len(ring), "input": "HBM-resident",
+                   "step_bytes": buf_bytes, "ring_buffers": a.ring, "input": "HBM-resident",
                    "dumps": "device-resident"},
         "roofline": roofline,
         "pipeline_roofline": pipeline,
         "kernels": kernels,
         "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3), "mode": "submit->complete, one buffer"},
         "cpu_baseline": cpu,
+        "configs": configs,
+        "serving": serve,
     }
     res.update(extra)
     print(json.dumps(res), flush=True)

@@ -62,6 +62,25 @@ def test_int16_ics_multi_antenna():
          conj=1)
 
 
+def test_config4_product_set_sixteen_antennas():
+    """The batched multi-antenna product set at full block length: 16
+    antennas x 4 coarse channels each (the full geometry has 16 each; the
+    channels are independent), 2 pols of complex int16 ~ round(N(0, 1000^2)),
+    blocks of 262144 samples, Nfft {64, 4096, 131072, 8} x Nint {2048, 32, 4,
+    16384}, pols 1, incoherent sum with weights uniform in [0.5, 1.5]; two
+    buffers so the 131072-point product (two buffers per dump) dumps too."""
+    import random
+    rng = random.Random(16)
+    w = [rng.uniform(0.5, 1.5) for _ in range(16)]
+    res, cfg, ev = _run(Nc=64, Nant=16, Ntpb=262144, Nbps=16, sigma=1000.0,
+                        Nts=[64, 4096, 131072, 8], Nas=[2048, 32, 4, 16384],
+                        Npolout=[1, 1, 1, 1], nbuf=2, ics=True, Aws=w)
+    assert [cfg.Nds[i] for i in range(4)] == [2, 2, 1, 2]
+    assert [cfg.Nis[i] for i in range(4)] == [1, 1, 2, 1]
+    assert sorted((n, i) for (n, i, *_r) in res) == [(0, 0), (0, 1), (0, 3), (1, 0), (1, 1),
+                                                      (1, 2), (1, 3)]
+
+
 def test_single_pol():
     _run(Np=1, Nc=4, Ntpb=32768, Nts=[32768, 256], Nas=[2, 16], nbuf=4)
 
