@@ -38,6 +38,20 @@ size_t rawspec_amd_device_input_size(rawspec_context *ctx);
 int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in,
                                         int fft_dir);
 
+/* As nbuf calls of rawspec_amd_start_processing_device(), one per buffer of
+ * d_in[0 .. nbuf-1] in order: the same integrations, the same dumps, each
+ * product's PRE/POST callbacks in the same order.  The device work is
+ * batched: a product whose integration spans several buffers transforms
+ * consecutive buffers together and adds them into its running sum once per
+ * batch (up to 8 buffers per launch) instead of once per buffer.  Two
+ * differences from the sequence of single calls: callbacks of different
+ * products may interleave differently (each product's own order is kept),
+ * and a batched sum adds the buffers in a different order, so results agree
+ * to FP32 rounding rather than bit for bit.  Every d_in[k] must stay
+ * unchanged until the work completes. */
+int rawspec_amd_start_processing_device_batch(rawspec_context *ctx, const void *const *d_in,
+                                              unsigned int nbuf, int fft_dir);
+
 /* enable != 0: dumps stay in device memory (no copy into h_pwrbuf/h_icsbuf);
  * the PRE/POST callbacks still fire in stream order.  The dumped spectra of
  * product i (same layout as h_pwrbuf[i]) are then valid at
@@ -50,6 +64,16 @@ float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i);
 /* The context's HIP stream (a hipStream_t), for callers that order their
  * own device work after the context's. */
 void *rawspec_amd_stream(rawspec_context *ctx);
+
+/* Transform engine of the context.  enable = 0: FP32 on the vector units
+ * (the default); enable != 0: the matrix cores (FP16 MFMA with hi/lo split
+ * operands and FP32 accumulation, whose error stays at the FP32 level)
+ * wherever a matrix-core kernel exists for the product's shape, the vector
+ * units elsewhere.  A product switches engine at the start of its next
+ * integration, so a partial integration is never mixed.  The getter returns
+ * the requested setting, -1 before initialisation. */
+int rawspec_amd_set_mfma(rawspec_context *ctx, int enable);
+int rawspec_amd_get_mfma(rawspec_context *ctx);
 
 /* Kernel timing.  With enable != 0 every kernel the context launches is
  * bracketed by HIP events on its stream.  rawspec_amd_profile_report()

@@ -118,11 +118,15 @@ def lib():
             ("rawspec_amd_device_input", [P], C.c_void_p),
             ("rawspec_amd_device_input_size", [P], C.c_size_t),
             ("rawspec_amd_start_processing_device", [P, C.c_void_p, C.c_int], C.c_int),
+            ("rawspec_amd_start_processing_device_batch",
+             [P, C.POINTER(C.c_void_p), C.c_uint, C.c_int], C.c_int),
             ("rawspec_amd_set_device_output", [P, C.c_int], C.c_int),
             ("rawspec_amd_device_pwrbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_device_icsbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_stream", [P], C.c_void_p),
             ("rawspec_amd_profile", [P, C.c_int], C.c_int),
+            ("rawspec_amd_set_mfma", [P, C.c_int], C.c_int),
+            ("rawspec_amd_get_mfma", [P], C.c_int),
             ("rawspec_amd_profile_report", [P, C.c_char_p, C.c_size_t], C.c_int),
         ]:
             f = getattr(L, name)
@@ -213,6 +217,11 @@ class Spectrometer:
         return lib().rawspec_amd_start_processing_device(C.byref(self.ctx), C.c_void_p(dev_ptr),
                                                          fft_dir)
 
+    def start_processing_device_batch(self, dev_ptrs, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
+        arr = (C.c_void_p * len(dev_ptrs))(*dev_ptrs)
+        return lib().rawspec_amd_start_processing_device_batch(C.byref(self.ctx), arr,
+                                                               len(dev_ptrs), fft_dir)
+
     def set_device_output(self, enable: bool) -> int:
         return lib().rawspec_amd_set_device_output(C.byref(self.ctx), 1 if enable else 0)
 
@@ -227,6 +236,12 @@ class Spectrometer:
 
     def stream(self) -> int:
         return lib().rawspec_amd_stream(C.byref(self.ctx)) or 0
+
+    def set_mfma(self, enable: bool) -> int:
+        return lib().rawspec_amd_set_mfma(C.byref(self.ctx), 1 if enable else 0)
+
+    def get_mfma(self) -> int:
+        return lib().rawspec_amd_get_mfma(C.byref(self.ctx))
 
     def profile(self, enable: bool) -> int:
         return lib().rawspec_amd_profile(C.byref(self.ctx), 1 if enable else 0)

@@ -26,7 +26,7 @@ ARCH = os.environ.get("RAWSPEC_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
-HIP_SOURCES = ["rsd_runtime.hip", "k_small.hip", "k_fourstep.hip", "k_misc.hip"]
+HIP_SOURCES = ["rsd_runtime.hip", "k_small.hip", "k_fourstep.hip", "k_mfma.hip", "k_misc.hip"]
 C_SOURCES = ["rawspec_api.c"]
 
 HIP_FLAGS = [

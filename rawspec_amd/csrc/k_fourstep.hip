@@ -46,45 +46,53 @@ constexpr int wg_team_np(int n) {
 constexpr int wg_team(int n) { return wg_team_np<2>(n); }
 constexpr int tile_of(int n) { return wg_team(n) / (n / elems_fs(n)); }
 
-template <int N2, int NBPS, int NP>
+// Pass A for N = N1 * N2 with N1 = R1 * N2 (R1 = 1 or 2).  The four-step
+// twiddle W_N^e, e = n1 * k2 < N, is split as e = h * N1 + r:
+// W_N^(h * N1) = W_N2^h is the sub-transform's own table (already in LDS)
+// and W_N^r, r < N1 <= 4096, is a second LDS table filled from the first
+// N1 entries of the FP64-generated fine table; so each twiddle costs two LDS
+// reads and one complex product (two correctly rounded factors, as with the
+// global two-level table) instead of two scattered global gathers.
+template <int N2, int R1, int NBPS, int NP>
 __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
   constexpr int E = elems_fs(N2);
   constexpr int T = N2 / E;
   constexpr int TA = tile_of(N2);      // n1 values per workgroup
+  constexpr int N1C = R1 * N2;
+  constexpr int L1 = ilog2(N1C);
+  static_assert(RSK_LDS_TW, "pass A reads the sub-transform table from LDS");
+  static_assert((TA * team_stride<N2, NP>() + TwTab<N2>::LEN + N1C) * 8 <= kLdsBytes,
+                "pass A LDS budget");
   using S = Sample<NBPS, NP>;
   using W = typename S::word;
   __shared__ float2 lds[TA * team_stride<N2, NP>()];
   __shared__ float2 twl[TwTab<N2>::LEN];
-  const float2 *twp = tw_setup<N2, wg_team(N2)>(twl, a.tw);
+  __shared__ float2 twr[N1C];
+  for (int m = threadIdx.x; m < N1C; m += wg_team(N2)) twr[m] = a.twf[m];
+  const float2 *twp = tw_setup<N2, wg_team(N2)>(twl, a.tw);   // also waits for twr
 
-  const unsigned N1 = a.N1;
+  const unsigned N1 = N1C;
   const unsigned long long N = (unsigned long long)N1 * N2;
   const unsigned tiles = N1 / TA;
   const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned tile = l % tiles;
   const unsigned cs = l / tiles;
-  const unsigned sp = cs % a.Nss;
-  const unsigned cl = cs / a.Nss;      // channel within the chunk
+  const unsigned sp = cs % a.Nss;      // spectrum within its buffer
+  const unsigned cb = cs / a.Nss;
+  const unsigned b = cb % a.nbat;      // buffer within the batch
+  const unsigned cl = cb / a.nbat;     // channel within the chunk
+  const unsigned nsp = a.Nss * a.nbat; // spectra per channel in Y
   const int tid = threadIdx.x;
   const int sg = tid % TA;             // n1 slot: fastest across lanes
   const int j = tid / TA;
   const unsigned n1 = tile * TA + sg;
 
-  const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
+  const W *src = reinterpret_cast<const W *>(a.in[b] + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
                  (unsigned long long)sp * N + n1;
   float2 v[NP][E];
   W raw[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) raw[m] = src[(unsigned long long)(j + m * T) * N1];
-  // the four-step twiddles W_N^(n1*k2) of this thread's outputs, fetched
-  // while the transform runs
-  float2 tc[E], tf[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const unsigned e = n1 * (j + m * T);   // < N <= 2^24
-    tc[m] = a.twc[e >> kTwLog];
-    tf[m] = a.twf[e & (kTwN - 1)];
-  }
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     float2 o[NP];
@@ -94,12 +102,13 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
   }
   team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
 
-  // Y[cl][sp][k2 / TBy][n1][k2 % TBy][pol]
-  float2 *dst = a.work + ((unsigned long long)cl * a.Nss + sp) * N * NP;
+  // Y[cl][b * Nss + sp][k2 / TBy][n1][k2 % TBy][pol]
+  float2 *dst = a.work + ((unsigned long long)cl * nsp + b * a.Nss + sp) * N * NP;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const unsigned k2 = j + m * T;
-    const float2 w = cmul(tc[m], tf[m]);
+    const unsigned e = n1 * k2;            // < N <= 2^24
+    const float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
     const unsigned long long off = ((unsigned long long)(k2 / TBy) * N1 + n1) * TBy + (k2 % TBy);
     if constexpr (NP == 2) {
       const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(wg_team(N1)) void k_fs_b(FsArgs a) {
 
   for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
     // this workgroup's tile: N1 rows of TB k2 values, contiguous
-    const float2 *y = a.work + (((unsigned long long)cl * a.Nss + sp) * N +
+    const float2 *y = a.work + (((unsigned long long)cl * a.Nss * a.nbat + sp) * N +
                                 (unsigned long long)tile * N1 * TB) * NP;
     float2 v[NP][E];
 #pragma unroll
@@ -206,12 +215,21 @@ __global__ __launch_bounds__(wg_team(N1)) void k_fs_b(FsArgs a) {
 int fs_tile_a(int n2) { return (int)(wg_team(n2) / (n2 / elems_fs(n2))); }
 int fs_tile_b(int n1) { return (int)(wg_team(n1) / (n1 / elems_fs(n1))); }
 
+template <int N2, int R1, int NBPS>
+static int go_a_r(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
+  constexpr int WGT = wg_team(N2);
+  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a, TBy);
+  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a, TBy);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 template <int N2, int NBPS>
 static int go_a(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
-  constexpr int WGT = wg_team(N2);
-  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a, TBy);
-  else hipLaunchKernelGGL((k_fs_a<N2, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a, TBy);
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  if (a.N1 == (unsigned)N2) return go_a_r<N2, 1, NBPS>(np, a, nb, TBy, s);
+  if constexpr (2 * N2 <= kTwN) {
+    if (a.N1 == 2u * N2) return go_a_r<N2, 2, NBPS>(np, a, nb, TBy, s);
+  }
+  return 2;
 }
 
 template <int NBPS>
@@ -230,9 +248,11 @@ static int go_a_n(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_
 
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
-  const unsigned TBy = (unsigned)fs_tile_b((int)a.N1);
-  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.nch;
-  if (nb == 0 || nb > 0x7fffffffull || a.N2 % TBy != 0) return 2;
+  const unsigned TBy = a.tby;
+  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
+  if (nb == 0 || nb > 0x7fffffffull || TBy == 0 || a.N2 % TBy != 0 || a.nbat == 0 ||
+      a.nbat > (unsigned)kMaxBatch)
+    return 2;
   if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, TBy, s);
   if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, TBy, s);
   return 2;

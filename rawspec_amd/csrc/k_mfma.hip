@@ -1,0 +1,176 @@
+// k_mfma.hip -- spectrometer kernels whose transforms run on the gfx950
+// matrix cores (v_mfma_f32_32x32x16_f16 with hi/lo split operands; see
+// mfma_dft.h for the decomposition, lane layout and precision).
+//
+// k_fs_b_mfma: pass B of the four-step path for N1 = 1024 and two pols, the
+// same contract as k_fs_b (kernels.h: FsArgs, modes FS_HOST .. FS_LAST):
+// length-1024 DFTs over n1 for every k2, detection, integration of the dump
+// group's spectra, FFT-shifted host layout or the internal accumulator.
+// One wavefront owns one k2 column of one channel and dump group, so the
+// transform needs no LDS exchange and no barrier; a workgroup is four
+// wavefronts on four consecutive k2 and shares LDS only to write the host
+// layout in 16-byte runs.  Pass A writes Y untiled for it,
+// Y[cl][s][k2][n1][pol], so each wavefront's loads are contiguous.
+#include "fft_device.h"
+#include "kernels.h"
+#include "mfma_dft.h"
+
+namespace rsk {
+
+constexpr int kMfmaWaves = 4;                      // k2 columns per workgroup
+constexpr int kOutRow = 4 * 1024 + 32;             // floats per q in the staging area
+
+// Detected powers of one k2 column and spectrum (both pols transformed,
+// then detected into acc, which is overwritten when SET, else added to).
+template <int PMODE, bool SET>
+__device__ __forceinline__ void fs_b_column(const float4 *y, float s1, float gam,
+                                            const Dft1024K &K, float (&acc)[16][nq<PMODE>()]) {
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 31, h = lane >> 5;
+  float4 v[4][4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[kb][i] = y[col + 32 * (8 * kb + 4 * h + i)];
+  f16v xr[2], xi[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    float2 x[4][4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        x[kb][i] = p == 0 ? make_float2(v[kb][i].x * s1, v[kb][i].y * s1)
+                          : make_float2(v[kb][i].z * s1, v[kb][i].w * s1);
+    dft1024<true>(x, K, xr[p], xi[p]);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    if constexpr (SET) {
+#pragma unroll
+      for (int q = 0; q < nq<PMODE>(); ++q) acc[r][q] = 0.0f;
+    }
+    const float2 xx[2] = {make_float2(xr[0][r], xi[0][r]), make_float2(xr[1][r], xi[1][r])};
+    detect_add<PMODE, 2>(acc[r], xx, gam);
+  }
+}
+
+// ONE: the dump group is a single spectrum (na_buf == 1, e.g. one 2^20
+// spectrum per channel and buffer), so the running sum is not live during
+// the transforms and two wavefronts fit per SIMD; otherwise one.
+template <int PMODE, bool ONE>
+__global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : 1) void k_fs_b_mfma(FsArgs a) {
+  constexpr int N1 = 1024;
+  constexpr int Q = nq<PMODE>();
+  // LDS: the DFT constants while transforming, then the output staging area
+  constexpr int kStBytes = Q * kOutRow * 4;
+  constexpr int kLds = kStBytes > (int)sizeof(Dft1024K) ? kStBytes : (int)sizeof(Dft1024K);
+  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLds];
+  Dft1024K &K = *reinterpret_cast<Dft1024K *>(lds_raw);
+  float *st = reinterpret_cast<float *>(lds_raw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, h = lane >> 5;
+  const unsigned N2 = a.N2;
+  const unsigned long long N = (unsigned long long)N1 * N2;
+  const unsigned tiles = N2 / kMfmaWaves;
+  const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
+  const unsigned tile = l % tiles;
+  const unsigned cg = l / tiles;
+  const unsigned g = cg % a.Ng;
+  const unsigned cl = cg / a.Ng;
+  const unsigned c = a.c0 + cl;
+  const unsigned k2 = tile * kMfmaWaves + wv;
+
+  dft1024_consts(K, a.tw, 64 * kMfmaWaves);
+  __syncthreads();
+
+  float acc[16][Q];
+  const float s1 = a.mscale;
+  const float4 *y0 = reinterpret_cast<const float4 *>(a.work) +
+                     ((unsigned long long)cl * a.Nss * a.nbat + g * a.na_buf) * N +
+                     (unsigned long long)k2 * N1;
+  if constexpr (ONE) {
+    fs_b_column<PMODE, true>(y0, s1, a.gam, K, acc);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) acc[r][q] = 0.0f;
+    for (unsigned sp = 0; sp < a.na_buf; ++sp) fs_b_column<PMODE, false>(y0 + sp * N, s1, a.gam, K, acc);
+  }
+
+  // undo the operand scaling (X = 32 * result / s1) and normalise
+  const float sc = a.scale * (32.0f / s1) * (32.0f / s1);
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[r][q] *= sc;
+
+  // internal accumulator (integrations over several buffers): one
+  // contiguous [Q][16][64] block per wavefront
+  if (a.mode != FS_HOST) {
+    float *ib = a.acc_int + (((unsigned long long)g * a.Nc + c) * N2 + k2) * (Q * 16 * 64) + lane;
+    if (a.mode == FS_MID || a.mode == FS_LAST) {
+      float prev[16][Q];
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) prev[r][q] = ib[(q * 16 + r) * 64];
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[r][q] += prev[r][q];
+    }
+    if (a.mode == FS_FIRST || a.mode == FS_MID) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) ib[(q * 16 + r) * 64] = acc[r][q];
+      return;   // uniform across the workgroup: no wavefront reaches the barrier below
+    }
+  }
+  // host layout: bin k1 = c_r + 32 col goes to shifted row (k1 + 512) % 1024;
+  // stage [q][row][4 k2] in LDS (one pad float per 32 rows), then write 16
+  // bytes (the workgroup's 4 consecutive k2) per row
+  __syncthreads();   // every wavefront is done with the constants
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int k1 = 8 * (r >> 2) + 4 * h + (r & 3) + 32 * col;
+    const int row = (k1 + N1 / 2) & (N1 - 1);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) st[q * kOutRow + 4 * row + (row >> 5) + wv] = acc[r][q];
+  }
+  __syncthreads();
+  float *ob = a.acc + ((unsigned long long)g * Q * a.Nc + c) * N + (unsigned long long)tile * kMfmaWaves;
+  for (int t = threadIdx.x; t < Q * N1; t += 64 * kMfmaWaves) {
+    const int q = t / N1, row = t - q * N1;
+    const float *s = st + q * kOutRow + 4 * row + (row >> 5);
+    reinterpret_cast<float4 *>(ob + (unsigned long long)q * a.Nc * N + (unsigned long long)N2 * row)[0] =
+        make_float4(s[0], s[1], s[2], s[3]);
+  }
+}
+
+template <int PMODE>
+static int go_b_mfma(const FsArgs &a, unsigned nb, hipStream_t s) {
+  if (a.na_buf == 1)
+    hipLaunchKernelGGL((k_fs_b_mfma<PMODE, true>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_fs_b_mfma<PMODE, false>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
+
+int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {
+  if (!fs_b_mfma_ok((int)a.N1, np) || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f)) return 2;
+  const unsigned long long nb = (unsigned long long)(a.N2 / kMfmaWaves) * a.Ng * a.nch;
+  if (nb == 0 || nb > 0x7fffffffull) return 2;
+  switch (pmode) {
+    case 1: return go_b_mfma<1>(a, (unsigned)nb, s);
+    case 4: return go_b_mfma<4>(a, (unsigned)nb, s);
+    case -4: return go_b_mfma<-4>(a, (unsigned)nb, s);
+  }
+  return 2;
+}
+
+}  // namespace rsk

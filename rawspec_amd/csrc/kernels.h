@@ -49,9 +49,16 @@ struct SmallArgs {
 // per workgroup).
 enum { FS_HOST = 0, FS_FIRST = 1, FS_MID = 2, FS_LAST = 3 };
 
+// Buffers one pass A/B launch pair can take (batched processing of
+// consecutive buffers of one integration).
+constexpr int kMaxBatch = 8;
+
 struct FsArgs {
-  const uint8_t *in;
+  const uint8_t *in[kMaxBatch];   // input buffer of each batched buffer
+  unsigned nbat;                  // buffers in this launch (1 .. kMaxBatch)
   unsigned long long chan_stride;
+  // Nss: spectra per channel and buffer; na_buf, Ng: pass B's dump groups
+  // over the launch's nbat * Nss spectra per channel
   unsigned Nc, Nss, na_buf, Ng;
   unsigned c0, nch;               // channel chunk
   unsigned N1, N2;
@@ -61,8 +68,9 @@ struct FsArgs {
   float *acc;                     // host layout [Nds][Q][Nc][N]
   float *acc_int;                 // internal accumulator (Nis > 1)
   const float2 *tw;               // W_4096^m
-  const float2 *twf;              // W_N^i, i < 4096
-  const float2 *twc;              // W_N^(4096 h), h < N/4096
+  const float2 *twf;              // W_N^r, r < N1 (N1 <= 4096)
+  unsigned tby;                   // k2 tile of Y: fs_tile_b(N1) for k_fs_b, 1 for k_fs_b_mfma
+  float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
 };
 
 struct ReduceArgs {
@@ -92,6 +100,9 @@ int launch_small(int n, int nbps, int np, int pmode, const SmallArgs &a,
                  unsigned nblocks, hipStream_t s);
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s);
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
+// Matrix-core pass B (N1 = 1024, two pols); needs Y written with tby = 1.
+int fs_b_mfma_ok(int n1, int np);
+int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s);
 int launch_reduce(const ReduceArgs &a, hipStream_t s);
 int launch_ics(const IcsArgs &a, hipStream_t s);
 int launch_expand4(const uint8_t *stage, uint8_t *dst, unsigned Nc,

@@ -418,6 +418,23 @@ int rawspec_zero_blocks_to_gpu(rawspec_context *ctx, off_t dst_idx, size_t num_b
  * the copy.  Host callbacks stall the stream while they run, so none is
  * enqueued when the context has no dump_callback (the reference would call
  * nothing either). */
+/* Enqueues the dump of product i: ICS, PRE callback, copy to h_pwrbuf /
+ * h_icsbuf (unless dumps stay on the device), POST callback
+ * (rawspec_gpu.cu:1603-1709). */
+static int rs_dump(rawspec_context *ctx, rs_gpu_ctx *g, unsigned int i)
+{
+  if (ctx->incoherently_sum && rsd_run_ics(g->dev, (int)i)) return 1;
+  if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
+  if (!g->device_output) {
+    if (rsd_dump_to_host(g->dev, (int)i, ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i],
+                         ctx->incoherently_sum ? ctx->h_icsbuf[i] : NULL,
+                         ctx->h_pwrbuf_size[i] / ctx->Nant))
+      return 1;
+  }
+  if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->post[i])) return 1;
+  return 0;
+}
+
 static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
 {
   unsigned int i;
@@ -432,18 +449,38 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
       fflush(stderr);
       return 1;
     }
-    if (g->inbuf_count % g->Nis[i] == 0) {
-      if (ctx->incoherently_sum && rsd_run_ics(g->dev, (int)i)) return 1;
-      if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->pre[i])) return 1;
-      if (!g->device_output) {
-        if (rsd_dump_to_host(g->dev, (int)i, ctx->h_pwrbuf[i], ctx->h_pwrbuf_size[i],
-                             ctx->incoherently_sum ? ctx->h_icsbuf[i] : NULL,
-                             ctx->h_pwrbuf_size[i] / ctx->Nant))
-          return 1;
+    if (last && rs_dump(ctx, g, i)) return 1;
+  }
+  return 0;
+}
+
+/* nbuf consecutive buffers: per product, the run is cut at integration
+ * boundaries; each piece goes to the device layer in one call (which
+ * batches it where the product's kernels support it), followed by the dump
+ * when the piece completes an integration. */
+static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigned int nbuf,
+                          int fft_dir)
+{
+  unsigned int i;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  for (i = 0; i < ctx->No; i++) {
+    unsigned int k = 0;
+    while (k < nbuf) {
+      const unsigned int pos = (g->inbuf_count + k) % g->Nis[i];   /* buffers into the integration */
+      const unsigned int left = g->Nis[i] - pos;
+      const unsigned int len = left < nbuf - k ? left : nbuf - k;
+      const int last = len == left;
+      if (rsd_run_product_batch(g->dev, (int)i, d_in + k, len, fft_dir <= 0, pos == 0, last)) {
+        fprintf(stderr, "librawspec: processing output product %u failed\n", i);
+        fflush(stderr);
+        return 1;
       }
-      if (ctx->dump_callback && rsd_host_fn(g->dev, rs_cb_trampoline, &g->post[i])) return 1;
+      if (last && rs_dump(ctx, g, i)) return 1;
+      k += len;
     }
   }
+  g->inbuf_count += nbuf;
   return 0;
 }
 
@@ -516,6 +553,23 @@ int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in, 
   return rs_start(ctx, d_in, fft_dir);
 }
 
+int rawspec_amd_start_processing_device_batch(rawspec_context *ctx, const void *const *d_in,
+                                              unsigned int nbuf, int fft_dir)
+{
+  unsigned int k;
+  if (!d_in || nbuf == 0) {
+    rs_err("rawspec_amd_start_processing_device_batch: need at least one input buffer");
+    return 1;
+  }
+  for (k = 0; k < nbuf; k++) {
+    if (!d_in[k] || ((size_t)d_in[k] & 15) != 0) {
+      rs_err("rawspec_amd_start_processing_device_batch: inputs must be 16-byte aligned device pointers");
+      return 1;
+    }
+  }
+  return rs_start_batch(ctx, d_in, nbuf, fft_dir);
+}
+
 int rawspec_amd_set_device_output(rawspec_context *ctx, int enable)
 {
   rs_gpu_ctx *g = rs_get(ctx);
@@ -549,6 +603,20 @@ int rawspec_amd_profile(rawspec_context *ctx, int enable)
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
   return rsd_profile(g->dev, enable);
+}
+
+int rawspec_amd_set_mfma(rawspec_context *ctx, int enable)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  return rsd_set_mfma(g->dev, enable);
+}
+
+int rawspec_amd_get_mfma(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return -1;
+  return rsd_get_mfma(g->dev);
 }
 
 int rawspec_amd_profile_report(rawspec_context *ctx, char *buf, size_t len)

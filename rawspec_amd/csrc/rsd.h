@@ -65,6 +65,10 @@ int  rsd_zero_block(rsd_ctx *d, unsigned dblk);
  * context's own input buffer).  first != 0 starts a new integration;
  * last != 0 completes one (the dump follows). */
 int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last);
+/* As rsd_run_product over n consecutive buffers d_in[0 .. n-1] of one
+ * integration (first: d_in[0] starts it; last: d_in[n-1] completes it). */
+int  rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n, int inverse,
+                           int first, int last);
 /* Enqueues the incoherent sum of product i's integration buffer. */
 int  rsd_run_ics(rsd_ctx *d, int i);
 /* Enqueues the device -> host copy of product i's dump (and ICS dump when
@@ -93,6 +97,12 @@ const char *rsd_engine_name(void);
  * clears the records.  Returns the number of bytes written, or -1. */
 int rsd_profile(rsd_ctx *d, int enable);
 int rsd_profile_report(rsd_ctx *d, char *buf, size_t len);
+
+/* Transform engine: 0 = FP32 vector units (default), 1 = matrix cores
+ * (split-FP16 MFMA) where a kernel exists for the shape.  A product switches
+ * at the start of its next integration. */
+int rsd_set_mfma(rsd_ctx *d, int enable);
+int rsd_get_mfma(rsd_ctx *d);
 
 #ifdef __cplusplus
 }

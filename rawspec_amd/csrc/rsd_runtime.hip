@@ -49,8 +49,10 @@ struct ProdPlan {
   int direct;
   // four-step path
   unsigned N1, N2;
-  unsigned fs_chunk;       // channels per pass A/B launch pair
-  float2 *d_twf, *d_twc;
+  unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
+  unsigned fs_bmax;        // buffers per batched launch pair (1: no batching)
+  int fs_mfma;             // pass B on the matrix cores for the current integration
+  float2 *d_twf;           // W_N^r, r < N1 (pass A's fine twiddle table)
   float *d_acc_int;        // running sum across buffers (Nis > 1)
   // outputs
   float *d_acc;
@@ -65,6 +67,12 @@ struct ProdPlan {
 #define RSD_FS_CACHE_MB 96
 #endif
 constexpr size_t kFsCacheBytes = (size_t)RSD_FS_CACHE_MB << 20;
+// Intermediate budget of a batched launch pair (several buffers' spectra
+// per channel): large enough to keep a few channels per launch.
+#ifndef RSD_FS_BATCH_MB
+#define RSD_FS_BATCH_MB 384
+#endif
+constexpr size_t kFsBatchBytes = (size_t)RSD_FS_BATCH_MB << 20;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -101,6 +109,7 @@ struct rsd_ctx {
   float *d_w;
   ProdPlan pp[RSD_MAX_PRODUCTS];
   // kernel timing (rsd_profile)
+  int mfma;                            // matrix-core engine requested (rsd_set_mfma)
   bool prof;
   struct Rec { std::string name; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -212,7 +221,6 @@ void rsd_destroy(rsd_ctx *d) {
     if (pp.d_acc) (void)hipFree(pp.d_acc);
     if (pp.d_ics) (void)hipFree(pp.d_ics);
     if (pp.d_twf) (void)hipFree(pp.d_twf);
-    if (pp.d_twc) (void)hipFree(pp.d_twc);
     if (pp.d_acc_int) (void)hipFree(pp.d_acc_int);
   }
   if (d->d_in) (void)hipFree(d->d_in);
@@ -228,6 +236,15 @@ void rsd_destroy(rsd_ctx *d) {
   for (auto e : d->pool) (void)hipEventDestroy(e);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
+}
+
+// Channels per pass A/B launch pair for launches of kb buffers: one buffer's
+// intermediate stays within kFsCacheBytes, so pass B reads it back from the
+// Infinity Cache; a batch's within kFsBatchBytes.
+static unsigned fs_chunk_for(const rsd_ctx *d, const ProdPlan &pp, unsigned kb) {
+  const size_t per_chan = (size_t)kb * pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
+  const size_t ch = (kb == 1 ? kFsCacheBytes : kFsBatchBytes) / per_chan;
+  return (unsigned)(ch < 1 ? 1 : (ch > d->cfg.Nc ? d->cfg.Nc : ch));
 }
 
 // Sizes and launch plans for a configuration, without touching the device.
@@ -267,13 +284,21 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
       pp.N2 = 1u << (L / 2);
       pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
       pp.Ng = pp.p.Nss / pp.na_buf;
-      // channels per A/B launch pair: the chunk's intermediate stays within
-      // kFsCacheBytes so pass B reads it back from the Infinity Cache
-      const size_t per_chan = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
-      size_t ch = kFsCacheBytes / per_chan;
-      pp.fs_chunk = (unsigned)(ch < 1 ? 1 : (ch > cfg->Nc ? cfg->Nc : ch));
-      const size_t wl = (size_t)pp.fs_chunk * pp.p.Nss * N * cfg->Np;
-      if (wl > work_max) work_max = wl;
+      pp.fs_chunk = fs_chunk_for(d, pp, 1);
+      // batches of consecutive buffers only where an integration spans
+      // several buffers (then each dump group is the whole buffer)
+      pp.fs_bmax = 1;
+      if (pp.p.Nis > 1) {
+        const size_t per_buf = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
+        size_t bm = kFsBatchBytes / per_buf;
+        bm = bm < 1 ? 1 : bm;
+        bm = bm > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : bm;
+        pp.fs_bmax = (unsigned)(bm > pp.p.Nis ? pp.p.Nis : bm);
+      }
+      for (unsigned kb = 1; kb <= pp.fs_bmax; ++kb) {
+        const size_t wl = (size_t)fs_chunk_for(d, pp, kb) * kb * pp.p.Nss * N * cfg->Np;
+        if (wl > work_max) work_max = wl;
+      }
     }
     const size_t pn = part_need(d, pp);
     if (pn > part_max) part_max = pn;
@@ -293,7 +318,7 @@ static size_t plan_bytes(const rsd_ctx *d) {
     const ProdPlan &pp = d->pp[i];
     b += pp.acc_len * sizeof(float) + pp.ics_len * sizeof(float);
     if (!pp.small) {
-      b += (rsk::kTwN + pp.p.Nt / rsk::kTwN) * sizeof(float2);
+      b += (size_t)pp.N1 * sizeof(float2);
       if (pp.p.Nis > 1) b += pp.acc_len * sizeof(float);
     }
   }
@@ -322,12 +347,9 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
     ProdPlan &pp = d->pp[i];
     const unsigned N = pp.p.Nt;
     if (!pp.small) {
-      std::vector<float2> f = twiddles(N, rsk::kTwN, 1);
-      std::vector<float2> c = twiddles(N, N / rsk::kTwN, rsk::kTwN);
+      std::vector<float2> f = twiddles(N, pp.N1, 1);
       RSD_CHECK(hipMalloc(&pp.d_twf, f.size() * sizeof(float2)));
       RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
-      RSD_CHECK(hipMalloc(&pp.d_twc, c.size() * sizeof(float2)));
-      RSD_CHECK(hipMemcpy(pp.d_twc, c.data(), c.size() * sizeof(float2), hipMemcpyHostToDevice));
       if (pp.p.Nis > 1 && alloc_zero((void **)&pp.d_acc_int, pp.acc_len * sizeof(float))) return 1;
     }
     if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
@@ -348,6 +370,7 @@ static rsd_ctx *new_ctx() {
   d->d_work = d->d_tw = nullptr;
   d->d_w = nullptr;
   d->prof = false;
+  d->mfma = 0;
   return d;
 }
 
@@ -397,6 +420,9 @@ int rsd_zero_block(rsd_ctx *d, unsigned dblk) {
   RSD_CHECK(hipStreamSynchronize(d->stream));
   return 0;
 }
+
+static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n, float isign,
+                        float gam, float scale, int first, int last);
 
 int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last) {
   RSD_CHECK(hipSetDevice(d->dev));
@@ -453,42 +479,92 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
     }
     return 0;
   }
+  return run_fourstep(d, i, &in, 1, isign, gam, scale, first, last);
+}
+
+// Four-step product i over n consecutive buffers of one integration: launch
+// pairs of up to fs_bmax buffers (batched: pass B sums their spectra before
+// touching the running sum), each over channel chunks sized so the chunk's Y
+// stays within its budget.
+static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n, float isign,
+                        float gam, float scale, int first, int last) {
+  ProdPlan &pp = d->pp[i];
+  const rsd_config &c = d->cfg;
+  const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
   rsk::FsArgs a;
-  a.in = in;
   a.chan_stride = d->chan_stride;
   a.Nc = c.Nc;
   a.Nss = pp.p.Nss;
-  a.na_buf = pp.na_buf;
-  a.Ng = pp.Ng;
   a.N1 = pp.N1;
   a.N2 = pp.N2;
   a.isign = isign;
   a.gam = gam;
   a.scale = scale;
-  if (pp.p.Nis == 1) a.mode = rsk::FS_HOST;
-  else if (first) a.mode = rsk::FS_FIRST;
-  else if (last) a.mode = rsk::FS_LAST;
-  else a.mode = rsk::FS_MID;
   a.work = d->d_work;
   a.acc = pp.d_acc;
   a.acc_int = pp.d_acc_int;
   a.tw = d->d_tw;
   a.twf = pp.d_twf;
-  a.twc = pp.d_twc;
-  // pass A and pass B alternate per channel chunk (cache-resident Y)
-  for (unsigned c0 = 0; c0 < c.Nc; c0 += pp.fs_chunk) {
-    a.c0 = c0;
-    a.nch = c.Nc - c0 < pp.fs_chunk ? c.Nc - c0 : pp.fs_chunk;
-    {
-      ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
-      if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+  // The engine is chosen when an integration starts and kept until it
+  // completes (the two pass-B kernels lay out the internal accumulator
+  // differently).
+  if (first) pp.fs_mfma = d->mfma && rsk::fs_b_mfma_ok((int)pp.N1, (int)c.Np);
+  a.tby = pp.fs_mfma ? 1u : (unsigned)rsk::fs_tile_b((int)pp.N1);
+  {
+    // largest |Y| is N2 * full * sqrt(2); scale it to at most 2^15
+    int e = 0;
+    while (ldexp((double)pp.N2 * full * 1.4142135623730951, -e) > 32768.0) ++e;
+    a.mscale = (float)ldexp(1.0, -e);
+  }
+  for (unsigned j = 0; j < n;) {
+    const unsigned kb = n - j < pp.fs_bmax ? n - j : pp.fs_bmax;
+    const bool f = first && j == 0, l = last && j + kb == n;
+    if (pp.p.Nis == 1 || (f && l)) a.mode = rsk::FS_HOST;
+    else if (f) a.mode = rsk::FS_FIRST;
+    else if (l) a.mode = rsk::FS_LAST;
+    else a.mode = rsk::FS_MID;
+    a.nbat = kb;
+    for (unsigned b = 0; b < (unsigned)rsk::kMaxBatch; ++b) a.in[b] = ins[j + (b < kb ? b : 0)];
+    a.na_buf = kb * pp.na_buf;
+    a.Ng = pp.Ng;
+    const unsigned chc = fs_chunk_for(d, pp, kb);
+    // pass A and pass B alternate per channel chunk
+    for (unsigned c0 = 0; c0 < c.Nc; c0 += chc) {
+      a.c0 = c0;
+      a.nch = c.Nc - c0 < chc ? c.Nc - c0 : chc;
+      {
+        ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
+        if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+      }
+      if (pp.fs_mfma) {
+        ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt);
+        if (rsk::launch_fs_b_mfma((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+      } else {
+        ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
+        if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+      }
     }
-    {
-      ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
-      if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
-    }
+    j += kb;
   }
   return 0;
+}
+
+int rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n, int inverse,
+                          int first, int last) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  ProdPlan &pp = d->pp[i];
+  if (n == 0) return 1;
+  if (pp.small || pp.fs_bmax <= 1) {
+    for (unsigned j = 0; j < n; ++j)
+      if (rsd_run_product(d, i, d_in[j], inverse, first && j == 0, last && j == n - 1)) return 1;
+    return 0;
+  }
+  const rsd_config &c = d->cfg;
+  const float isign = inverse ? -1.0f : 1.0f;
+  const float gam = (c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f);
+  const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
+  return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, isign, gam,
+                      1.0f / (full * full), first, last);
 }
 
 int rsd_run_ics(rsd_ctx *d, int i) {
@@ -552,6 +628,13 @@ int rsd_profile(rsd_ctx *d, int enable) {
   d->prof = enable != 0;
   return 0;
 }
+
+int rsd_set_mfma(rsd_ctx *d, int enable) {
+  d->mfma = enable != 0;
+  return 0;
+}
+
+int rsd_get_mfma(rsd_ctx *d) { return d->mfma; }
 
 int rsd_profile_report(rsd_ctx *d, char *buf, size_t len) {
   RSD_CHECK(hipSetDevice(d->dev));
