@@ -8,12 +8,17 @@ with a complex tone of amplitude 20 at fine bin 300/1024 of channel 3
 per device buffer; three products Nfft = {1048576, 8, 1024} integrating
 Nint = {51, 128, 3072} spectra, full Stokes (pols = -4), forward FFT.
 
-One step = one device buffer through rawspec_amd_start_processing_device()
-(the same kernels rawspec_start_processing() runs), with the input already
-resident in HBM: a ring of 64 buffers (16 GiB, the whole 128-block run) is
-generated on the device before timing and cycled.  Dumps stay in device
-memory (rawspec_amd_set_device_output), so PCIe is outside the timed
-region; the PCIe-inclusive figure is reported separately (--pcie).
+One step = one device buffer, with the input already resident in HBM: a
+ring of 64 buffers (16 GiB, the whole 128-block run) is generated on the
+device before timing and cycled.  Dumps stay in device memory
+(rawspec_amd_set_device_output), so PCIe is outside the timed region; the
+PCIe-inclusive figure is reported separately (--pcie).  Three paths are
+timed: "plain", one buffer per rawspec_amd_start_processing_device() call
+(the granularity of the reference's rawspec_start_processing()) on the FP32
+vector units; "served", batches of 8 consecutive buffers per
+rawspec_amd_start_processing_device_batch() call; and "served_mfma", the
+same on the matrix-core engine.  `value` is the faster served path (both
+keep the plain path's error bar).
 
 Also measured, reported under "configs": the other single-stream workloads
 (config 1: 8 ch, 4 MiB buffers, Nfft 1024; config 2: config 3's geometry
@@ -119,23 +124,36 @@ def alg_bytes(sp, buf_bytes):
     return buf_bytes + out, per_prod
 
 
-def timed(torch, sp, ring, steps, warmup):
-    def step(k):
-        if sp.start_processing_device(ring[k % len(ring)].data_ptr(), 1):
-            raise RuntimeError("rawspec_amd_start_processing_device failed")
+def submitter(sp, ring, batch):
+    """go(k0, n): submits buffers k0 .. k0+n-1 of the ring (n a multiple of
+    batch), one call per buffer (batch 1, rawspec_amd_start_processing_device)
+    or per `batch` consecutive buffers (rawspec_amd_start_processing_device_batch)."""
+    ptrs = [r.data_ptr() for r in ring]
 
-    for k in range(warmup):
-        step(k)
+    def go(k0, n):
+        for k in range(k0, k0 + n, batch):
+            if batch == 1:
+                rc = sp.start_processing_device(ptrs[k % len(ptrs)], 1)
+            else:
+                rc = sp.start_processing_device_batch(
+                    [ptrs[(k + j) % len(ptrs)] for j in range(batch)], 1)
+            if rc:
+                raise RuntimeError("start processing failed")
+    return go
+
+
+def timed(torch, sp, ring, steps, warmup, batch=1):
+    go = submitter(sp, ring, batch)
+    go(0, warmup)
     if sp.wait_for_completion():
         raise RuntimeError("wait failed")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(steps):
-        step(warmup + k)
+    go(warmup, steps)
     if sp.wait_for_completion():
         raise RuntimeError("wait failed")
     torch.cuda.synchronize()
-    return time.perf_counter() - t0, step
+    return time.perf_counter() - t0, go
 
 
 def cpu_baseline(sp, ring, nthreads, nbuf, chans):
@@ -182,7 +200,9 @@ def serving(torch):
     out = {"streams": len(reqs), "arrivals": "Poisson, mean 4/s, seed 5",
            "ingest_bytes": sum(r.nbuf * S.plan(r.shape).buf_bytes for r in reqs)}
     for name, kw, scale in [("concurrent", {}, 1.0), ("sequential", {"max_active": 1}, 1.0),
+                            ("concurrent_batched", {"batch": 8}, 1.0),
                             ("concurrent_all_present", {}, 0.0),
+                            ("concurrent_batched_all_present", {"batch": 8}, 0.0),
                             ("sequential_all_present", {"max_active": 1}, 0.0)]:
         st = S.Server(**kw).run(reqs, arrival_scale=scale)
         st.pop("results")
@@ -196,8 +216,10 @@ def serving(torch):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=510)
-    ap.add_argument("--warmup", type=int, default=102)
+    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--warmup", type=int, default=104)
+    ap.add_argument("--batch", type=int, default=8,
+                    help="buffers per call on the served path (rawspec_amd_start_processing_device_batch)")
     ap.add_argument("--ring", type=int, default=64, help="device-resident input buffers")
     ap.add_argument("--latency-steps", type=int, default=102)
     ap.add_argument("--cpu-buffers", type=int, default=8)
@@ -208,6 +230,8 @@ def main(argv=None):
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
     a = ap.parse_args(argv)
 
+    a.steps = max(a.batch, (a.steps + a.batch - 1) // a.batch * a.batch)
+    a.warmup = (a.warmup + a.batch - 1) // a.batch * a.batch
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
@@ -227,34 +251,48 @@ def main(argv=None):
     ring = gen_ring(torch, sp, W, a.ring)
     log(f"bench: ring of {len(ring)} device buffers x {buf_bytes / 2**20:.0f} MiB ready")
 
-    dt, step = timed(torch, sp, ring, a.steps, a.warmup)
-    value = a.steps * buf_bytes / dt / 1e9
-    ms_per_step = dt / a.steps * 1e3
-    log(f"bench: {a.steps} steps in {dt:.3f} s -> {value:.1f} GB/s ({ms_per_step:.3f} ms/step)")
+    def measure(batch, mfma):
+        """Throughput of one leg, then its per-kernel HIP-event timing."""
+        sp.set_mfma(mfma)          # takes effect as each product starts an integration
+        dt, go = timed(torch, sp, ring, a.steps, a.warmup, batch=batch)
+        sp.profile(True)
+        nprof = max(batch, (max(51, min(a.steps, 152)) // batch) * batch)
+        go(0, nprof)
+        prof = sp.profile_report()
+        sp.profile(False)
+        kern = {k: {"launches": n, "avg_ms": ms / n, "total_ms": ms, "ms_per_step": ms / nprof}
+                for k, (n, ms) in prof.items()}
+        gbs = a.steps * buf_bytes / dt / 1e9
+        log(f"bench: batch {batch} mfma {int(mfma)}: {a.steps} steps in {dt:.3f} s -> {gbs:.1f} GB/s "
+            f"({dt / a.steps * 1e3:.3f} ms/step); " +
+            ", ".join(f"{k}={v['ms_per_step']:.3f}ms/step" for k, v in sorted(kern.items())))
+        return {"GBps": gbs, "ms_per_step": dt / a.steps * 1e3, "kernels": kern, "nprof": nprof,
+                "go": go}
 
-    # per-kernel HIP-event timing on the context's stream (separate pass)
-    sp.profile(True)
-    nprof = max(51, min(a.steps, 153))
-    for k in range(nprof):
-        step(k)
-    prof = sp.profile_report()
-    sp.profile(False)
-    kernels = {k: {"launches": n, "avg_ms": ms / n, "total_ms": ms} for k, (n, ms) in prof.items()}
+    legs = {"plain": measure(1, False), "served": measure(a.batch, False),
+            "served_mfma": measure(a.batch, True)}
+    sp.set_mfma(False)
+    head_name = max(("served", "served_mfma"), key=lambda k: legs[k]["GBps"])
+    head = legs[head_name]
+    value, ms_per_step = head["GBps"], head["ms_per_step"]
+    kernels = head["kernels"]
+    step_go = legs["plain"]["go"]
+
+    # roofline of the headline leg's dominant kernel: algorithmic bytes per
+    # launch = its share of the RAW bytes read once and of the dump bytes
+    # written once per integration, split evenly over its launches per step
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     B_alg, per_prod = alg_bytes(sp, buf_bytes)
     prod = int(dom.split(".")[0][1:])
     kname = dom.split(".")[1]
     out_b, nis = per_prod[prod]                     # dump bytes of that product, buffers per dump
-    # algorithmic bytes of the kernel per step: RAW bytes it must read and
-    # dump bytes it must write (amortised over Nis); split evenly over its
-    # launches per step (the four-step passes run once per channel chunk)
-    if kname in ("k_small",):
+    if kname.startswith("k_small"):
         kb_step = buf_bytes + out_b / nis
     elif kname == "k_fs_a":
         kb_step = buf_bytes
     else:
         kb_step = out_b / nis
-    kb = kb_step / (kernels[dom]["launches"] / nprof)
+    kb = kb_step / (kernels[dom]["launches"] / head["nprof"])
     dom_ms = kernels[dom]["avg_ms"]
     achieved = kb / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -262,19 +300,24 @@ def main(argv=None):
                 "kernel": dom, "alg_bytes_per_launch": kb, "avg_launch_ms": dom_ms}
     pipeline = {"alg_bytes_per_step": B_alg, "achieved_GBs": round(B_alg / (ms_per_step * 1e-3) / 1e9, 1),
                 "frac": round(B_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    log(f"bench: dominant kernel {dom} avg {dom_ms:.3f} ms; kernels: " +
-        ", ".join(f"{k}={v['avg_ms']:.3f}ms x{v['launches']}" for k, v in sorted(kernels.items())))
+    log(f"bench: headline {head_name}; dominant kernel {dom} avg {dom_ms:.3f} ms")
 
-    # synchronous per-buffer latency (submit -> all products complete)
-    lat = []
-    for k in range(a.latency_steps):
-        t1 = time.perf_counter()
-        step(k)
-        sp.wait_for_completion()
-        lat.append((time.perf_counter() - t1) * 1e3)
-    lat.sort()
-    p50 = lat[len(lat) // 2]
-    p99 = lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))]
+    # synchronous latency: submit -> every product complete, one buffer
+    # (plain) and one batch (served)
+    def latency(go, batch):
+        lat = []
+        for k in range(0, a.latency_steps * batch, batch):
+            t1 = time.perf_counter()
+            go(k, batch)
+            sp.wait_for_completion()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        lat.sort()
+        return {"p50": round(lat[len(lat) // 2], 3),
+                "p99": round(lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], 3)}
+
+    lat_plain = latency(step_go, 1)
+    lat_served = latency(legs["served"]["go"], a.batch)
+    p50, p99 = lat_plain["p50"], lat_plain["p99"]
 
     extra = {}
     if a.pcie:
@@ -319,13 +362,20 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": "config3: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
                                "blocks of 524288 samples, Nb=2 (256 MiB/step), "
-                               "Nfft {1048576,8,1024} x Nint {51,128,3072}, pols -4, plain FP32 path",
+                               "Nfft {1048576,8,1024} x Nint {51,128,3072}, pols -4",
+                   "path": f"{head_name}: batches of {a.batch} consecutive buffers per call, "
+                           f"{'matrix-core' if head_name == 'served_mfma' else 'FP32 vector'} engine",
                    "step_bytes": buf_bytes, "ring_buffers": a.ring, "input": "HBM-resident",
                    "dumps": "device-resident"},
+        "paths": {k: {"GBps": round(v["GBps"], 2), "ms_per_step": round(v["ms_per_step"], 4),
+                      "vs_plain": round(v["GBps"] / legs["plain"]["GBps"], 3)}
+                  for k, v in legs.items()},
         "roofline": roofline,
         "pipeline_roofline": pipeline,
         "kernels": kernels,
-        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3), "mode": "submit->complete, one buffer"},
+        "kernels_plain": legs["plain"]["kernels"],
+        "latency_ms": {"plain": dict(lat_plain, mode="submit->complete, one buffer"),
+                       "served": dict(lat_served, mode=f"submit->complete, one batch of {a.batch}")},
         "cpu_baseline": cpu,
         "configs": configs,
         "serving": serve,

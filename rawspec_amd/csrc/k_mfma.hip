@@ -2,6 +2,8 @@
 // matrix cores (v_mfma_f32_32x32x16_f16 with hi/lo split operands; see
 // mfma_dft.h for the decomposition, lane layout and precision).
 //
+// k_small_mfma: the fused kernel for N = 1024 (below).
+//
 // k_fs_b_mfma: pass B of the four-step path for N1 = 1024 and two pols, the
 // same contract as k_fs_b (kernels.h: FsArgs, modes FS_HOST .. FS_LAST):
 // length-1024 DFTs over n1 for every k2, detection, integration of the dump
@@ -55,11 +57,17 @@ __device__ __forceinline__ void fs_b_column(const float4 *y, float s1, float gam
   }
 }
 
+// Wavefronts per SIMD the loop variant of k_fs_b_mfma is built for (the
+// running sum is live across its transforms).
+#ifndef RSK_FSB_LOOP_WAVES
+#define RSK_FSB_LOOP_WAVES 2
+#endif
+
 // ONE: the dump group is a single spectrum (na_buf == 1, e.g. one 2^20
 // spectrum per channel and buffer), so the running sum is not live during
-// the transforms and two wavefronts fit per SIMD; otherwise one.
+// the transforms; otherwise a loop over the group's spectra.
 template <int PMODE, bool ONE>
-__global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : 1) void k_fs_b_mfma(FsArgs a) {
+__global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void k_fs_b_mfma(FsArgs a) {
   constexpr int N1 = 1024;
   constexpr int Q = nq<PMODE>();
   // LDS: the DFT constants while transforming, then the output staging area
@@ -148,6 +156,134 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : 1) void k_fs_b_mfma(FsAr
     reinterpret_cast<float4 *>(ob + (unsigned long long)q * a.Nc * N + (unsigned long long)N2 * row)[0] =
         make_float4(s[0], s[1], s[2], s[3]);
   }
+}
+
+// k_small_mfma: the fused small-N kernel (k_small's contract, SmallArgs)
+// for N = 1024 with two pols: unpack -> 1024-point DFT on the matrix cores
+// -> detection -> integration in registers -> FFT-shifted output.  One
+// wavefront integrates one chunk of consecutive spectra of one coarse
+// channel (item = (channel, group, chunk)); four independent wavefronts per
+// workgroup share the DFT constants in LDS, and the same LDS then stages
+// each wavefront's sums so the output is written in 256-byte runs.  Integer
+// samples are exact in FP16 up to 2048, so int8 input skips the lo operand
+// of the first stage.
+constexpr int kSmRow = 1024 + 32;                  // staged floats per (wave, q)
+
+template <int NBPS, int PMODE>
+__global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2) void k_small_mfma(SmallArgs a, unsigned items) {
+  constexpr int N = 1024;
+  constexpr int Q = nq<PMODE>();
+  constexpr bool LO = NBPS == 16;
+  using S = Sample<NBPS, 2>;
+  using W = typename S::word;
+  constexpr int kStBytes = kMfmaWaves * Q * kSmRow * 4;
+  constexpr int kLds = kStBytes > (int)sizeof(Dft1024K) ? kStBytes : (int)sizeof(Dft1024K);
+  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLds];
+  Dft1024K &K = *reinterpret_cast<Dft1024K *>(lds_raw);
+  float *st = reinterpret_cast<float *>(lds_raw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, h = lane >> 5;
+  const unsigned item = blockIdx.x * kMfmaWaves + wv;
+  const bool valid = item < items;
+  // item -> (channel, group, chunk) as k_small's (c, gb, chunk) with gpw = 1
+  const unsigned it = valid ? item : 0u;
+  const unsigned c = it / (a.Ng * a.G);
+  const unsigned rest = it - c * (a.Ng * a.G);
+  const unsigned g = rest / a.G;
+  const unsigned chunk = rest - g * a.G;
+
+  dft1024_consts(K, a.tw, 64 * kMfmaWaves);
+  __syncthreads();
+
+  float acc[16][Q];
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[r][q] = 0.0f;
+  const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)c * a.chan_stride) +
+                 ((unsigned long long)g * a.na_buf + (unsigned long long)chunk * a.chunk) * N;
+  const unsigned nspec = valid ? a.chunk : 0u;
+  for (unsigned s = 0; s < nspec; ++s) {
+    W raw[4][4];
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) raw[kb][i] = src[(unsigned long long)s * N + col + 32 * (8 * kb + 4 * h + i)];
+    f16v xr[2], xi[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float2 x[4][4];
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float2 o[2];
+          S::cvt(raw[kb][i], o, a.isign);
+          x[kb][i] = o[p];
+        }
+      dft1024<LO>(x, K, xr[p], xi[p]);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const float2 xx[2] = {make_float2(xr[0][r], xi[0][r]), make_float2(xr[1][r], xi[1][r])};
+      detect_add<PMODE, 2>(acc[r], xx, a.gam);
+    }
+  }
+
+  // the transforms return X / 32: sums are 1024 x too small
+  __syncthreads();   // every wavefront is done with the constants
+  float *sw = st + wv * (Q * kSmRow);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int k = 8 * (r >> 2) + 4 * h + (r & 3) + 32 * col;
+    const int jo = (k + N / 2) & (N - 1);            // FFT-shifted fine channel
+#pragma unroll
+    for (int q = 0; q < Q; ++q) sw[q * kSmRow + jo + (jo >> 5)] = acc[r][q] * 1024.0f;
+  }
+  __syncthreads();
+  if (!valid) return;
+  for (int q = 0; q < Q; ++q) {
+    float *dst;
+    if (a.direct) dst = a.acc + (((unsigned long long)g * Q + q) * a.Nc + c) * N;
+    else dst = a.part + ((((unsigned long long)c * a.Ng + g) * a.G + chunk) * Q + q) * N;
+#pragma unroll 4
+    for (int jo = lane; jo < N; jo += 64) {
+      float v = sw[q * kSmRow + jo + (jo >> 5)];
+      if (a.direct) {
+        v *= a.scale;
+        if (!a.first) v += dst[jo];
+      }
+      dst[jo] = v;
+    }
+  }
+}
+
+template <int NBPS, int PMODE>
+static int go_small_mfma(const SmallArgs &a, unsigned items, hipStream_t s) {
+  const unsigned nb = (items + kMfmaWaves - 1) / kMfmaWaves;
+  hipLaunchKernelGGL((k_small_mfma<NBPS, PMODE>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a, items);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int small_mfma_ok(int n, int np) { return n == 1024 && np == 2; }
+
+int launch_small_mfma(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
+                      hipStream_t s) {
+  if (!small_mfma_ok(n, np) || items == 0 || a.gpw != 1) return 2;
+  if (nbps == 8) {
+    switch (pmode) {
+      case 1: return go_small_mfma<8, 1>(a, items, s);
+      case 4: return go_small_mfma<8, 4>(a, items, s);
+      case -4: return go_small_mfma<8, -4>(a, items, s);
+    }
+  } else if (nbps == 16) {
+    switch (pmode) {
+      case 1: return go_small_mfma<16, 1>(a, items, s);
+      case 4: return go_small_mfma<16, 4>(a, items, s);
+      case -4: return go_small_mfma<16, -4>(a, items, s);
+    }
+  }
+  return 2;
 }
 
 template <int PMODE>

@@ -100,6 +100,11 @@ int launch_small(int n, int nbps, int np, int pmode, const SmallArgs &a,
                  unsigned nblocks, hipStream_t s);
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s);
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
+// Matrix-core fused small kernel (N = 1024, two pols): one wavefront per
+// (channel, group, chunk) item, items = Nc * Ng * G, gpw must be 1.
+int small_mfma_ok(int n, int np);
+int launch_small_mfma(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
+                      hipStream_t s);
 // Matrix-core pass B (N1 = 1024, two pols); needs Y written with tby = 1.
 int fs_b_mfma_ok(int n1, int np);
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s);

@@ -84,8 +84,7 @@ __device__ __forceinline__ void dft1024_consts(Dft1024K &k, const float2 *tw4096
 // FP16 operands per K-block.  LO = false when A is exact in FP16.
 template <bool LO>
 __device__ __forceinline__ void dft32_stage(const h8v (&ah)[4], const h8v (&al)[4],
-                                            const Dft1024K &k, f16v &cr, f16v &ci) {
-  const int lane = threadIdx.x & 63;
+                                            const Dft1024K &k, int lane, f16v &cr, f16v &ci) {
 #pragma unroll
   for (int r = 0; r < 16; ++r) { cr[r] = 0.0f; ci[r] = 0.0f; }
 #pragma unroll
@@ -109,7 +108,12 @@ __device__ __forceinline__ void dft32_stage(const h8v (&ah)[4], const h8v (&al)[
 template <bool LO>
 __device__ __forceinline__ void dft1024(const float2 (&x)[4][4], const Dft1024K &k, f16v &xr,
                                         f16v &xi) {
-  const int lane = threadIdx.x & 63;
+  // The constants are re-read from LDS for every transform: an opaque zero
+  // in the index keeps the compiler from hoisting them out of the caller's
+  // loop into 96 registers per lane, which would halve the occupancy.
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  const int lane = (threadIdx.x & 63) + z;
   h8v ah[4], al[4];
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb)
@@ -124,7 +128,7 @@ __device__ __forceinline__ void dft1024(const float2 (&x)[4][4], const Dft1024K 
       }
     }
   f16v zr, zi;
-  dft32_stage<LO>(ah, al, k, zr, zi);
+  dft32_stage<LO>(ah, al, k, lane, zr, zi);
   // twiddle, scale by 1/32 (|Z| <= 2^20 -> <= 2^15), split for stage 2
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -134,7 +138,7 @@ __device__ __forceinline__ void dft1024(const float2 (&x)[4][4], const Dft1024K 
     set_hl(ah[r >> 2], al[r >> 2], 2 * (r & 3), wr);
     set_hl(ah[r >> 2], al[r >> 2], 2 * (r & 3) + 1, wi);
   }
-  dft32_stage<true>(ah, al, k, xr, xi);
+  dft32_stage<true>(ah, al, k, lane, xr, xi);
 }
 
 }  // namespace rsk

@@ -9,10 +9,11 @@
 //           dump is one contiguous device->host copy
 //   d_ics   per product, [Nds][|P|][Nc/Nant][Nt] float
 //   d_part  partial integrations of groups split over workgroups
-//   d_work  four-step intermediate [Nc][Nss][N1][N2][Np] float2
-//   d_tw    W_4096^m table; per four-step product W_N^i (i < 4096) and
-//           W_N^(4096 h) tables.  All twiddles are computed in FP64 on the
-//           host and rounded once to FP32.
+//   d_work  four-step intermediate of one channel chunk (several buffers'
+//           spectra when batched), tiled for pass B
+//   d_tw    W_4096^m table; per four-step product the fine table W_N^r,
+//           r < N1.  All twiddles are computed in FP64 on the host and
+//           rounded once to FP32.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -47,6 +48,9 @@ struct ProdPlan {
   // small path
   unsigned na_buf, Ng, chunk, G, gpw, spg, bpc;
   int direct;
+  // matrix-core small kernel (N = 1024, two pols): one wavefront per chunk
+  int m_ok;
+  unsigned m_chunk, m_G;
   // four-step path
   unsigned N1, N2;
   unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
@@ -61,18 +65,16 @@ struct ProdPlan {
   size_t ics_len;
 };
 
-// Four-step intermediate budget per pass A/B launch pair: well inside the
-// MI355X's 256 MB Infinity Cache so pass B's reads hit it.
-#ifndef RSD_FS_CACHE_MB
-#define RSD_FS_CACHE_MB 96
+// Four-step intermediate budget per pass A/B launch pair (one buffer's or a
+// batch's spectra for a chunk of channels).  Larger launches keep more of
+// the device busy and outweigh keeping the intermediate inside the 256 MB
+// Infinity Cache: measured on 64 channels x 2^20 points, one launch pair over
+// all channels (1 GiB) runs 8% faster than 6-channel (96 MiB) chunks, and a
+// batch of 8 buffers x 12 channels 6% faster than 8 x 3.
+#ifndef RSD_FS_WORK_MB
+#define RSD_FS_WORK_MB 1536
 #endif
-constexpr size_t kFsCacheBytes = (size_t)RSD_FS_CACHE_MB << 20;
-// Intermediate budget of a batched launch pair (several buffers' spectra
-// per channel): large enough to keep a few channels per launch.
-#ifndef RSD_FS_BATCH_MB
-#define RSD_FS_BATCH_MB 384
-#endif
-constexpr size_t kFsBatchBytes = (size_t)RSD_FS_BATCH_MB << 20;
+constexpr size_t kFsWorkBytes = (size_t)RSD_FS_WORK_MB << 20;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -179,9 +181,25 @@ static void plan_small(rsd_ctx *d, ProdPlan &pp) {
   pp.bpc = ((pp.Ng + pp.gpw - 1) / pp.gpw) * pp.G;
 }
 
+// Matrix-core small kernel plan: halve the chunks until about 2048
+// wavefronts (eight per CU) are busy, keeping >= 8 spectra per wavefront.
+static void plan_small_mfma(rsd_ctx *d, ProdPlan &pp) {
+  pp.m_ok = rsk::small_mfma_ok((int)pp.p.Nt, (int)d->cfg.Np);
+  pp.m_chunk = pp.na_buf;
+  while ((unsigned long long)d->cfg.Nc * pp.Ng * (pp.na_buf / pp.m_chunk) < 2048 &&
+         pp.m_chunk % 2 == 0 && pp.m_chunk / 2 >= 8)
+    pp.m_chunk /= 2;
+  pp.m_G = pp.na_buf / pp.m_chunk;
+}
+
 static size_t part_need(const rsd_ctx *d, const ProdPlan &pp) {
-  if (!pp.small || pp.direct) return 0;
-  return (size_t)d->cfg.Nc * pp.Ng * pp.G * pp.Q * pp.p.Nt;
+  if (!pp.small) return 0;
+  size_t n = pp.direct ? 0 : (size_t)d->cfg.Nc * pp.Ng * pp.G * pp.Q * pp.p.Nt;
+  if (pp.m_ok && pp.m_G > 1) {
+    const size_t m = (size_t)d->cfg.Nc * pp.Ng * pp.m_G * pp.Q * pp.p.Nt;
+    if (m > n) n = m;
+  }
+  return n;
 }
 
 static int alloc_zero(void **p, size_t bytes) {
@@ -238,12 +256,11 @@ void rsd_destroy(rsd_ctx *d) {
   delete d;
 }
 
-// Channels per pass A/B launch pair for launches of kb buffers: one buffer's
-// intermediate stays within kFsCacheBytes, so pass B reads it back from the
-// Infinity Cache; a batch's within kFsBatchBytes.
+// Channels per pass A/B launch pair for launches of kb buffers: as many as
+// keep the chunk's intermediate within kFsWorkBytes (at least one).
 static unsigned fs_chunk_for(const rsd_ctx *d, const ProdPlan &pp, unsigned kb) {
   const size_t per_chan = (size_t)kb * pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
-  const size_t ch = (kb == 1 ? kFsCacheBytes : kFsBatchBytes) / per_chan;
+  const size_t ch = kFsWorkBytes / per_chan;
   return (unsigned)(ch < 1 ? 1 : (ch > d->cfg.Nc ? d->cfg.Nc : ch));
 }
 
@@ -278,6 +295,7 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
     pp.small = N <= (unsigned)rsk::kTwN;
     if (pp.small) {
       plan_small(d, pp);
+      plan_small_mfma(d, pp);
     } else {
       const unsigned L = ilog2u(N);
       pp.N1 = 1u << ((L + 1) / 2);
@@ -290,7 +308,7 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
       pp.fs_bmax = 1;
       if (pp.p.Nis > 1) {
         const size_t per_buf = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
-        size_t bm = kFsBatchBytes / per_buf;
+        size_t bm = kFsWorkBytes / per_buf;
         bm = bm < 1 ? 1 : bm;
         bm = bm > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : bm;
         pp.fs_bmax = (unsigned)(bm > pp.p.Nis ? pp.p.Nis : bm);
@@ -455,21 +473,35 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
     a.acc = pp.d_acc;
     a.part = d->d_part;
     a.tw = d->d_tw;
-    const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
-    if (nb > 0x7fffffffull) return 1;
-    {
+    // Both engines write the same (host) layout, so a product can change
+    // engine between any two buffers.
+    const bool mf = d->mfma && pp.m_ok;
+    if (mf) {
+      a.chunk = pp.m_chunk;
+      a.G = pp.m_G;
+      a.gpw = 1;
+      a.direct = pp.m_G == 1;
+      const unsigned long long items = (unsigned long long)c.Nc * pp.Ng * pp.m_G;
+      if (items > 0x7fffffffull) return 1;
+      ProfScope ps(d, i, "k_small_mfma", pp.p.Nt);
+      if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
+                                 (unsigned)items, d->stream))
+        return 1;
+    } else {
+      const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
+      if (nb > 0x7fffffffull) return 1;
       ProfScope ps(d, i, "k_small", pp.p.Nt);
       if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
                             d->stream))
         return 1;
     }
-    if (!pp.direct) {
+    if (!a.direct) {
       rsk::ReduceArgs r;
       r.part = d->d_part;
       r.acc = pp.d_acc;
       r.Nc = c.Nc;
       r.Ng = pp.Ng;
-      r.G = pp.G;
+      r.G = a.G;
       r.Q = (unsigned)pp.Q;
       r.N = pp.p.Nt;
       r.scale = scale;

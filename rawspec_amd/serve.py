@@ -12,8 +12,9 @@ coexist in one process).  The loop does continuous batching, not queueing:
     rawspec_amd_plan() reports plus its input ring against the HBM budget,
     and the pinned host bytes against the host budget; requests wait in
     arrival order otherwise;
-  * every admitted stream keeps up to `depth` buffers queued on its HIP
-    stream, refilled as soon as one completes, interleaved with all others;
+  * every admitted stream keeps up to `depth` submissions (each of `batch`
+    consecutive buffers) queued on its HIP stream, refilled as soon as one
+    completes, interleaved with all others;
   * a finished stream is retired at once and its memory returns to the
     budgets for the next arrival.
 
@@ -175,7 +176,7 @@ class _Stream:
         import torch
         self.req, self.plan = req, plan_
         self.t_arrival, self.t_admit = t_arrival, t_admit
-        self.depth = srv.depth
+        self.depth, self.batch = srv.depth, srv.batch
         self.sent = 0
         self.t_sub = []
         self.inflight = collections.deque()
@@ -186,6 +187,8 @@ class _Stream:
                                callback=self._on_dump, record_events=False)
         if self.sp.set_device_output(not srv.record_dumps):
             raise RuntimeError("rawspec_amd_set_device_output failed")
+        if self.sp.set_mfma(srv.mfma):
+            raise RuntimeError("rawspec_amd_set_mfma failed")
         assert self.sp.device_input_size() == plan_.buf_bytes
         self.hs = torch.cuda.ExternalStream(self.sp.stream(), device=f"cuda:{srv.gpu}")
         self.ring = srv.make_ring(req, plan_)
@@ -210,14 +213,18 @@ class _Stream:
             self.inflight.popleft()
             moved = True
         while len(self.inflight) < self.depth and self.sent < self.req.nbuf:
-            self.t_sub.append(time.perf_counter())
-            buf = self.ring[self.sent % len(self.ring)]
-            if self.sp.start_processing_device(buf.data_ptr(), 1):
+            nb = min(self.batch, self.req.nbuf - self.sent)
+            t = time.perf_counter()
+            self.t_sub.extend([t] * nb)
+            ptrs = [self.ring[(self.sent + j) % len(self.ring)].data_ptr() for j in range(nb)]
+            rc = (self.sp.start_processing_device(ptrs[0], 1) if nb == 1
+                  else self.sp.start_processing_device_batch(ptrs, 1))
+            if rc:
                 raise RuntimeError(f"stream {self.req.sid}: start processing failed")
             ev = torch.cuda.Event()
             ev.record(self.hs)
             self.inflight.append(ev)
-            self.sent += 1
+            self.sent += nb
             moved = True
         return moved
 
@@ -238,17 +245,21 @@ class Server:
     hbm_budget / host_budget: bytes the admitted streams may hold together
     on the device (contexts plus input rings) and in pinned host memory.
     max_active: concurrent streams (slots); 1 runs the streams one after the
-    other.  depth: buffers queued per stream.  ring: distinct device input
-    buffers per stream (buffer k reads ring[k % ring]).  stream_cls: the
+    other.  depth: submissions queued per stream; batch: buffers per
+    submission (1: rawspec_amd_start_processing_device, more: the batched
+    entry point).  mfma: transform engine of every stream's context.  ring:
+    distinct device input buffers per stream (buffer k reads
+    ring[k % ring]).  stream_cls: the
     class that serves one admitted request (its constructor, pump(),
     finished, close(), sent, lat, dumps, plan); the scheduler itself touches
     no device.
     """
 
     def __init__(self, hbm_budget=200e9, host_budget=96 * GiB, max_active=32, depth=2, ring=2,
-                 gpu=0, record_dumps=False, poll_s=20e-6, stream_cls=None):
+                 gpu=0, record_dumps=False, poll_s=20e-6, stream_cls=None, batch=1, mfma=False):
         self.hbm_budget, self.host_budget = hbm_budget, host_budget
         self.max_active, self.depth, self.nring = max_active, depth, ring
+        self.batch, self.mfma = max(1, int(batch)), bool(mfma)
         self.gpu, self.record_dumps, self.poll_s = gpu, record_dumps, poll_s
         self.stream_cls = stream_cls or _Stream
         self._gen = None

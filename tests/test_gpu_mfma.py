@@ -74,3 +74,27 @@ def test_mfma_and_vector_engines_agree():
     print(f"  vector {e_v:.2e}  matrix {e_m:.2e}  between {d:.2e}", flush=True)
     assert not np.array_equal(g_v, g_m)      # the matrix-core kernel really ran
     assert e_m <= max(PLAIN_TOL, MFMA_FLOOR_MULT * e_v)
+
+
+@pytest.mark.parametrize("mode", [1, 4, -4])
+@pytest.mark.parametrize("nint", [8, 64, 256])
+def test_mfma_small_1024(mode, nint):
+    # Nint 8: one wavefront per dump group, written directly; 64: groups split
+    # over wavefronts (partial sums + reduction); 256: integration over 4
+    # buffers of 64 spectra
+    _run(Nc=3, Ntpb=65536, Nts=[1024], Nas=[nint], Npolout=[mode], nbuf=4, conj=1,
+         fft_dir=-1 if mode == 4 else 1, tone=(1, 15.0, 37.0, 128.0))
+
+
+def test_mfma_small_1024_int16_with_other_lengths():
+    _run(Nc=4, Ntpb=16384, Nbps=16, sigma=1000.0, Nts=[1024, 4096, 8], Nas=[16, 4, 512],
+         Npolout=[-4, -4, 1], nbuf=3)
+
+
+def test_mfma_small_engine_switch_mid_integration():
+    # the small kernels of both engines share the host layout: switching in
+    # the middle of an integration is exact bookkeeping
+    res, cfg, ev = run_case(Nc=2, Ntpb=32768, Nts=[1024], Nas=[96], Npolout=[-4], nbuf=6,
+                            tone=(0, 20.0, 300.0, 1024.0), mfma_switch=(2, True))
+    check_results(res, cfg, tol=PLAIN_TOL)
+    assert len(res) == 2
