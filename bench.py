@@ -124,6 +124,21 @@ def alg_bytes(sp, buf_bytes):
     return buf_bytes + out, per_prod
 
 
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    the counter's reading for a 1 GiB copy), or None when none matches."""
+    path = os.path.join(ROOT, "profiles", "config3_hbm_traffic.json")
+    try:
+        rec = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    if not k or rec.get("batch") != batch:
+        return None
+    return k.get("bytes_per_launch")
+
+
 def submitter(sp, ring, batch):
     """go(k0, n): submits buffers k0 .. k0+n-1 of the ring (n a multiple of
     batch), one call per buffer (batch 1, rawspec_amd_start_processing_device)
@@ -228,6 +243,11 @@ def main(argv=None):
     ap.add_argument("--no-others", action="store_true", help="headline workload only")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
+    ap.add_argument("--legs", default="plain,served,served_mfma",
+                    help="paths to time (comma-separated subset of the default)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain 1 GiB device copy 4 times (calibrates the HBM "
+                         "byte counters of a rocprofv3 --pmc run of this command)")
     a = ap.parse_args(argv)
 
     a.steps = max(a.batch, (a.steps + a.batch - 1) // a.batch * a.batch)
@@ -269,14 +289,24 @@ def main(argv=None):
         return {"GBps": gbs, "ms_per_step": dt / a.steps * 1e3, "kernels": kern, "nprof": nprof,
                 "go": go}
 
-    legs = {"plain": measure(1, False), "served": measure(a.batch, False),
-            "served_mfma": measure(a.batch, True)}
+    if a.calib_copy:
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda").random_(0, 255)
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    want = [x for x in a.legs.split(",") if x]
+    spec = {"plain": (1, False), "served": (a.batch, False), "served_mfma": (a.batch, True)}
+    legs = {k: measure(*spec[k]) for k in spec if k in want}
+    if "plain" not in legs:
+        legs["plain"] = legs[want[0]]
     sp.set_mfma(False)
-    head_name = max(("served", "served_mfma"), key=lambda k: legs[k]["GBps"])
+    head_name = max([k for k in ("served", "served_mfma") if k in legs] or ["plain"],
+                    key=lambda k: legs[k]["GBps"])
     head = legs[head_name]
     value, ms_per_step = head["GBps"], head["ms_per_step"]
     kernels = head["kernels"]
-    step_go = legs["plain"]["go"]
 
     # roofline of the headline leg's dominant kernel: algorithmic bytes per
     # launch = its share of the RAW bytes read once and of the dump bytes
@@ -296,7 +326,8 @@ def main(argv=None):
     dom_ms = kernels[dom]["avg_ms"]
     achieved = kb / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom, a.batch if head_name != "plain" else 1),
                 "kernel": dom, "alg_bytes_per_launch": kb, "avg_launch_ms": dom_ms}
     pipeline = {"alg_bytes_per_step": B_alg, "achieved_GBs": round(B_alg / (ms_per_step * 1e-3) / 1e9, 1),
                 "frac": round(B_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -315,8 +346,10 @@ def main(argv=None):
         return {"p50": round(lat[len(lat) // 2], 3),
                 "p99": round(lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], 3)}
 
-    lat_plain = latency(step_go, 1)
-    lat_served = latency(legs["served"]["go"], a.batch)
+    lat_plain = latency(submitter(sp, ring, 1), 1)
+    sp.set_mfma(head_name == "served_mfma")
+    lat_served = latency(submitter(sp, ring, a.batch), a.batch)
+    sp.set_mfma(False)
     p50, p99 = lat_plain["p50"], lat_plain["p99"]
 
     extra = {}
