@@ -237,7 +237,7 @@ def main(argv=None):
                     help="buffers per call on the served path (rawspec_amd_start_processing_device_batch)")
     ap.add_argument("--ring", type=int, default=64, help="device-resident input buffers")
     ap.add_argument("--latency-steps", type=int, default=102)
-    ap.add_argument("--cpu-buffers", type=int, default=8)
+    ap.add_argument("--cpu-buffers", type=int, default=24)
     ap.add_argument("--cpu-chans", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="headline workload only")
@@ -370,7 +370,8 @@ def main(argv=None):
     cpu = None
     if not a.no_cpu:
         nth = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        chans = sorted({0, 3, c.Nc - 1} | set(range(5, c.Nc, max(1, c.Nc // a.cpu_chans))))[:a.cpu_chans]
+        chans = (list(range(c.Nc)) if a.cpu_chans >= c.Nc else
+                 sorted({0, 3, c.Nc - 1} | set(range(5, c.Nc, max(1, c.Nc // a.cpu_chans))))[:a.cpu_chans])
         v, cdt = cpu_baseline(sp, ring, nth, a.cpu_buffers, chans)
         cpu = {"value": round(v, 4), "unit": "GB/s", "cores": nth, "kind": "port",
                "sample": f"FP64 C oracle (OpenMP), {len(chans)} of {c.Nc} channels x "

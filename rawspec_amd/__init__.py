@@ -21,6 +21,7 @@ __all__ = [
     "MAX_OUTPUTS", "RAWSPEC_FORWARD_FFT", "RAWSPEC_INVERSE_FFT",
     "RAWSPEC_CALLBACK_PRE_DUMP", "RAWSPEC_CALLBACK_POST_DUMP",
     "RawspecContext", "DumpCallback", "lib", "lib_path", "Spectrometer",
+    "RawHdr", "FbHdr", "host_format_prototypes",
 ]
 
 MAX_OUTPUTS = 4
@@ -69,6 +70,86 @@ RawspecContext._fields_ = [
     ("Ntmax", C.c_uint),
     ("gpu_ctx", C.c_void_p),
 ]
+
+
+class RawHdr(C.Structure):
+    """rawspec_raw_hdr_t (include/rawspec_rawutils.h)."""
+    _fields_ = [
+        ("directio", C.c_int), ("blocsize", C.c_size_t), ("npol", C.c_uint),
+        ("obsnchan", C.c_uint), ("nbits", C.c_uint), ("pktidx", C.c_int64),
+        ("obsfreq", C.c_double), ("obsbw", C.c_double), ("tbin", C.c_double),
+        ("ra", C.c_double), ("dec", C.c_double), ("mjd", C.c_double),
+        ("beam_id", C.c_int), ("nbeam", C.c_int), ("refbeam", C.c_int), ("nants", C.c_uint),
+        ("src_name", C.c_char * 81), ("telescop", C.c_char * 81),
+        ("hdr_pos", C.c_long), ("hdr_size", C.c_size_t),
+    ]
+
+
+class FbHdr(C.Structure):
+    """fb_hdr_t (include/rawspec_fbutils.h)."""
+    _fields_ = [
+        ("machine_id", C.c_int), ("telescope_id", C.c_int), ("data_type", C.c_int),
+        ("barycentric", C.c_int), ("pulsarcentric", C.c_int),
+        ("src_raj", C.c_double), ("src_dej", C.c_double), ("az_start", C.c_double),
+        ("za_start", C.c_double), ("fch1", C.c_double), ("foff", C.c_double),
+        ("nchans", C.c_int), ("nbeams", C.c_int), ("ibeam", C.c_int), ("nbits", C.c_int),
+        ("tstart", C.c_double), ("tsamp", C.c_double), ("nifs", C.c_int),
+        ("source_name", C.c_char * 81), ("rawdatafile", C.c_char * 81),
+        ("nfpc", C.c_int), ("refbeam", C.c_int),
+    ]
+
+
+def host_format_prototypes(L, only=None):
+    """Sets ctypes prototypes of the GUPPI RAW (rawspec_raw_*) and SIGPROC
+    filterbank (fb_*) header functions on a loaded library; `only`, a name
+    prefix, restricts them to one family (for a library exporting just it)."""
+    cp, vp, i32p, dp = C.c_char_p, C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    RH, FH = C.POINTER(RawHdr), C.POINTER(FbHdr)
+    for name, args, res in [
+        ("rawspec_raw_get_s32", [cp, cp, C.c_int32], C.c_int32),
+        ("rawspec_raw_get_u32", [cp, cp, C.c_uint32], C.c_uint32),
+        ("rawspec_raw_get_s64", [cp, cp, C.c_int64], C.c_int64),
+        ("rawspec_raw_get_u64", [cp, cp, C.c_uint64], C.c_uint64),
+        ("rawspec_raw_get_dbl", [cp, cp, C.c_double], C.c_double),
+        ("rawspec_raw_get_str", [cp, cp, cp, vp, C.c_size_t], None),
+        ("rawspec_raw_dmsstr_to_d", [vp], C.c_double),
+        ("rawspec_raw_header_size", [vp, C.c_size_t, C.c_int], C.c_int),
+        ("rawspec_raw_parse_header", [cp, RH], None),
+        ("rawspec_raw_read_header", [C.c_int, RH], C.c_long),
+        ("fb_ddd_to_dms", [C.c_double], C.c_double),
+        ("fb_dms_to_ddd", [C.c_double], C.c_double),
+        ("fb_fd_write_int", [C.c_int, C.c_int32], C.c_ssize_t),
+        ("fb_buf_write_int", [vp, C.c_int32], vp),
+        ("fb_fd_write_double", [C.c_int, C.c_double], C.c_ssize_t),
+        ("fb_buf_write_double", [vp, C.c_double], vp),
+        ("fb_fd_write_angle", [C.c_int, C.c_double], C.c_ssize_t),
+        ("fb_buf_write_angle", [vp, C.c_double], vp),
+        ("fb_fd_write_string", [C.c_int, cp], C.c_ssize_t),
+        ("fb_buf_write_string", [vp, cp], vp),
+        ("fb_fd_read_int", [C.c_int, i32p], C.c_int32),
+        ("fb_buf_read_int", [vp, i32p], vp),
+        ("fb_fd_read_double", [C.c_int, dp], C.c_double),
+        ("fb_buf_read_double", [vp, dp], vp),
+        ("fb_fd_read_angle", [C.c_int, dp], C.c_double),
+        ("fb_buf_read_angle", [vp, dp], vp),
+        ("fb_fd_read_string", [C.c_int, vp, i32p], C.c_ssize_t),
+        ("fb_buf_peek_string", [vp, C.POINTER(C.c_void_p), i32p], vp),
+        ("fb_buf_read_string", [vp, vp, i32p], vp),
+        ("fb_fd_write_padded_header", [C.c_int, FH, C.c_int32], C.c_ssize_t),
+        ("fb_fd_write_header", [C.c_int, FH], C.c_ssize_t),
+        ("fb_buf_write_padded_header", [vp, FH, C.c_int32], vp),
+        ("fb_buf_write_header", [vp, FH], vp),
+        ("fb_fd_read_header", [C.c_int, FH, C.POINTER(C.c_size_t)], C.c_ssize_t),
+        ("fb_buf_read_header", [vp, FH, C.POINTER(C.c_size_t)], vp),
+        ("fb_telescope_id", [cp], C.c_int),
+    ]:
+        if only is not None and not name.startswith(only):
+            continue
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
+
 
 _lib = None
 
@@ -132,6 +213,7 @@ def lib():
             f = getattr(L, name)
             f.argtypes = args
             f.restype = res
+        host_format_prototypes(L)
         _lib = L
     return _lib
 

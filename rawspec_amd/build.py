@@ -27,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
 
 HIP_SOURCES = ["rsd_runtime.hip", "k_small.hip", "k_fourstep.hip", "k_mfma.hip", "k_misc.hip"]
-C_SOURCES = ["rawspec_api.c"]
+C_SOURCES = ["rawspec_api.c", "raw_header.c", "fb_header.c"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",

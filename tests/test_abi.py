@@ -50,10 +50,10 @@ def test_ctypes_mirror_matches_header(rlib):
 
 def _declared_functions():
     names = set()
-    for h in ("rawspec.h", "rawspec_amd.h"):
+    for h in ("rawspec.h", "rawspec_amd.h", "rawspec_rawutils.h", "rawspec_fbutils.h"):
         txt = open(os.path.join(INC, h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b((?:rawspec|get)_\w+)\s*\(", txt, re.M):
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b((?:rawspec|get|fb)_\w+)\s*\(", txt, re.M):
             names.add(m.group(1))
     return names
 
@@ -62,7 +62,8 @@ def test_library_exports_every_declared_function(rlib):
     from rawspec_amd.build import exported_symbols
     declared = _declared_functions()
     assert {"rawspec_initialize", "rawspec_cleanup", "rawspec_start_processing",
-            "rawspec_wait_for_completion", "get_cufft_version"} <= declared
+            "rawspec_wait_for_completion", "get_cufft_version", "rawspec_raw_read_header",
+            "fb_fd_write_padded_header", "fb_telescope_id"} <= declared
     exported = set(exported_symbols(rlib.lib_path()))
     missing = declared - exported
     assert not missing, missing
