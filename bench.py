@@ -25,7 +25,10 @@ Also measured, reported under "configs": the other single-stream workloads
 with pols 1 and pols 4; config 4: 16 antennas x 16 channels of complex
 int16, four products with the incoherent sum) and, under "serving", the
 serving loop (32 Poisson-arriving streams of random shapes, concurrent vs
-one at a time).
+one at a time); under "file_ingest", the rawspec program reading a GUPPI RAW
+file of the headline geometry from the page cache and writing filterbank
+files, with the reference program's read-copy-start order and with the
+pipelined ingest.
 
 Prints one JSON line.
 """
@@ -209,6 +212,69 @@ def other_config(torch, R, W, nring=8):
     return out
 
 
+def file_ingest(torch, W, nblocks):
+    """The rawspec program on a GUPPI RAW file of W's geometry (nblocks
+    blocks, DIRECTIO headers, samples from the seeded device generator, read
+    back from the page cache), in three ingest modes; plus the pinned
+    host-to-device copy rate, the PCIe bound of any file ingest."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from rawspec_amd.rawfile import header, observation_cards
+    cli = os.path.join(ROOT, "rawspec_amd", "bin", "rawspec")
+    if not os.path.exists(cli):
+        log("bench: rawspec program not built; no file ingest figures")
+        return None
+    d = tempfile.mkdtemp(prefix="rawspec_ingest_")
+    out = {"blocks": nblocks, "block_bytes": W["Nc"] * W["Ntpb"] * W["Np"] * 2 * W["Nbps"] // 8,
+           "source": "one GUPPI RAW file in the page cache", "output": "SIGPROC filterbank files"}
+    try:
+        stem = os.path.join(d, "stem")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(W["seed"])
+        lim = 32767 if W["Nbps"] == 16 else 127
+        dt = torch.int16 if W["Nbps"] == 16 else torch.int8
+        with open(stem + ".0000.raw", "wb") as f:
+            for b in range(nblocks):
+                cards = observation_cards(obsnchan=W["Nc"], npol=W["Np"], nbits=W["Nbps"],
+                                          ntpb=W["Ntpb"], pktidx=32 * b, directio=1)
+                f.write(header(cards, 1))
+                x = torch.randn((W["Nc"], W["Ntpb"], W["Np"], 2), generator=g, device="cuda")
+                f.write(x.mul_(W["sigma"]).round_().clamp_(-lim, lim).to(dt).cpu().numpy().tobytes())
+                del x
+        dest = os.path.join(d, "out")
+        os.mkdir(dest)
+        args = ["-f", ",".join(map(str, W["Nts"])), "-t", ",".join(map(str, W["Nas"])),
+                "-p", ",".join(map(str, W["Npolout"])), "-d", dest, "--stats"]
+        for name, extra in [("reference_order", ["--sync-ingest", "--readers", "1"]),
+                            ("sync_parallel_reads", ["--sync-ingest", "--readers", "16"]),
+                            ("pipelined", ["--readers", "16"])]:
+            r = subprocess.run([cli, *args, *extra, stem], capture_output=True, text=True,
+                               timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"rawspec failed: {r.stderr[-2000:]}")
+            st = json.loads([x for x in r.stderr.splitlines() if x.startswith("{")][-1])
+            out[name] = {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+            log(f"bench: file ingest {name}: {st['GBps']:.2f} GB/s "
+                f"({st['buffers']} buffers, read wait {st['read_wait_seconds']:.3f} s of "
+                f"{st['seconds']:.3f} s)")
+        n = out["block_bytes"] * 2
+        h = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        dv = torch.empty(n, dtype=torch.uint8, device="cuda")
+        dv.copy_(h, non_blocking=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(8):
+            dv.copy_(h, non_blocking=True)
+        torch.cuda.synchronize()
+        out["h2d_pinned_GBps"] = round(8 * n / (time.perf_counter() - t1) / 1e9, 2)
+        del h, dv
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def serving(torch):
     import rawspec_amd.serve as S
     reqs = S.make_requests(n=32, rate=4.0, seed=5)
@@ -242,6 +308,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="headline workload only")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving loop")
+    ap.add_argument("--ingest-blocks", type=int, default=32,
+                    help="blocks of the GUPPI RAW file the rawspec program ingests (0: skip)")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
     ap.add_argument("--legs", default="plain,served,served_mfma",
                     help="paths to time (comma-separated subset of the default)")
@@ -387,6 +455,7 @@ def main(argv=None):
         for Wo in OTHER:
             configs[Wo["name"]] = other_config(torch, R, Wo)
             log(f"bench: {Wo['name']}: {configs[Wo['name']]}")
+    ingest = file_ingest(torch, W, a.ingest_blocks) if a.ingest_blocks > 0 else None
     serve = None if a.no_serve else serving(torch)
 
     res = {
@@ -412,6 +481,7 @@ def main(argv=None):
                        "served": dict(lat_served, mode=f"submit->complete, one batch of {a.batch}")},
         "cpu_baseline": cpu,
         "configs": configs,
+        "file_ingest": ingest,
         "serving": serve,
     }
     res.update(extra)

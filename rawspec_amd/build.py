@@ -21,6 +21,9 @@ CSRC = os.path.join(HERE, "csrc")
 INC = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librawspec.so")
+CLI_SRC = os.path.join(CSRC, "cli")
+CLI = os.path.join(HERE, "bin", "rawspec")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 ARCH = os.environ.get("RAWSPEC_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -81,6 +84,29 @@ def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
     return out
 
 
+CLI_SOURCES = ["cli_main.c", "cli_ingest.c", "cli_output.c", "cli_net.c"]
+
+
+def build_cli(lib: str = LIB, out: str = CLI, verbose: bool = False) -> str:
+    """Compile the rawspec program (host C) against the built library; it
+    finds librawspec.so next to its own directory at run time."""
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    objs = []
+    for src in CLI_SOURCES:
+        o = os.path.join(BUILD, "cli_" + src + ".o")
+        _run([CC, "-c", *C_FLAGS, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{CLI_SRC}",
+              os.path.join(CLI_SRC, src), "-o", o])
+        objs.append(o)
+    tmp = out + ".tmp"
+    _run([CC, "-o", tmp, *objs, f"-L{os.path.dirname(lib)}", "-lrawspec",
+          "-Wl,-rpath,$ORIGIN/..", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
+          "-lpthread", "-lm"])
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 def exported_symbols(path: str = LIB):
     nm = shutil.which("nm") or "nm"
     r = _run([nm, "-D", "--defined-only", path])
@@ -93,6 +119,7 @@ def main(argv=None):
     ap.add_argument("-j", type=int, default=None)
     a = ap.parse_args(argv)
     build(a.j, verbose=True)
+    build_cli(verbose=True)
     if a.check:
         for s in exported_symbols():
             if s.startswith(("rawspec", "get_")):
