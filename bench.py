@@ -216,7 +216,9 @@ def file_ingest(torch, W, nblocks):
     """The rawspec program on a GUPPI RAW file of W's geometry (nblocks
     blocks, DIRECTIO headers, samples from the seeded device generator, read
     back from the page cache), in three ingest modes; plus the pinned
-    host-to-device copy rate, the PCIe bound of any file ingest."""
+    host-to-device copy rate, the PCIe bound of any file ingest.  Modes:
+    the reference program's order (one reader; read, wait, copy, start),
+    the same order with 16 read threads, and the pipelined ingest."""
     import shutil
     import subprocess
     import tempfile
@@ -247,11 +249,12 @@ def file_ingest(torch, W, nblocks):
         os.mkdir(dest)
         args = ["-f", ",".join(map(str, W["Nts"])), "-t", ",".join(map(str, W["Nas"])),
                 "-p", ",".join(map(str, W["Npolout"])), "-d", dest, "--stats"]
+        # the stem four times over (integrations restart per stem): 4 x nblocks blocks
         for name, extra in [("reference_order", ["--sync-ingest", "--readers", "1"]),
                             ("sync_parallel_reads", ["--sync-ingest", "--readers", "16"]),
-                            ("pipelined", ["--readers", "16"])]:
-            r = subprocess.run([cli, *args, *extra, stem], capture_output=True, text=True,
-                               timeout=600)
+                            ("pipelined", [])]:
+            r = subprocess.run([cli, *args, *extra, stem, stem, stem, stem], capture_output=True,
+                               text=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(f"rawspec failed: {r.stderr[-2000:]}")
             st = json.loads([x for x in r.stderr.splitlines() if x.startswith("{")][-1])
@@ -269,6 +272,8 @@ def file_ingest(torch, W, nblocks):
             dv.copy_(h, non_blocking=True)
         torch.cuda.synchronize()
         out["h2d_pinned_GBps"] = round(8 * n / (time.perf_counter() - t1) / 1e9, 2)
+        out["pipelined_vs_reference_order"] = round(out["pipelined"]["GBps"] /
+                                                    out["reference_order"]["GBps"], 2)
         del h, dv
     finally:
         shutil.rmtree(d, ignore_errors=True)

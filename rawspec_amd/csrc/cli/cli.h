@@ -94,10 +94,15 @@ void cli_ingest_destroy(cli_ingest *g);
 int cli_ingest_block(cli_ingest *g, unsigned long bi, int fd, off_t off);
 /* Block bi is zeros (a gap in PKTIDX). */
 int cli_ingest_zero_block(cli_ingest *g, unsigned long bi);
-/* Buffer complete (bi % Nb == Nb - 1): read it and start processing.
+/* Buffer complete (bi % Nb == Nb - 1): read it and start processing (in
+ * pipelined mode: start reading it, and start the previous buffer).
  * Returns non-zero on a read error or when processing could not start
  * (*fatal set for the latter). */
 int cli_ingest_submit(cli_ingest *g, int *fatal);
+/* Starts the last complete buffer still held back (pipelined mode starts a
+ * buffer when the next one is complete); call at the end of a stem.
+ * Returns as cli_ingest_submit. */
+int cli_ingest_flush(cli_ingest *g, int *fatal);
 /* Drops the blocks of an incomplete buffer (end of a stem). */
 void cli_ingest_discard(cli_ingest *g);
 /* Bytes read from files and buffers started so far. */

@@ -76,8 +76,8 @@ static void usage(const char *argv0)
           "  -t, --ints=N1[,N2...]  Spectra to integrate [51, 128, 3072]\n"
           "  -z, --debug            Turn on selected debug output\n"
           "\n"
-          "      --readers=N        Threads reading blocks [8]\n"
-          "      --slots=N          Pinned staging buffers for pipelined ingest [3]\n"
+          "      --readers=N        Threads reading blocks [12]\n"
+          "      --slots=N          Pinned staging buffers for pipelined ingest [2]\n"
           "      --sync-ingest      Read, copy and start each buffer in turn (the library's\n"
           "                         standard copy API) instead of the pipelined ingest\n"
           "      --mfma             Use the matrix-core transform engine\n"
@@ -148,7 +148,7 @@ int main(int argc, char *argv[])
   int ant = -1, opt, i, si, exit_status = 0;
   unsigned int schan = 0, nchan = 0, outidx = 0;
   double rate = 6.0;
-  int readers = 8, nslots = 3, sync_ingest = 0, mfma = 0, stats = 0;
+  int readers = 12, nslots = 2, sync_ingest = 0, mfma = 0, stats = 0;
   unsigned int Nc = 0, Ncpa, Np = 0, Ntpb = 0, Nbps = 0;
   int expand4 = 0;
   double t_start = 0, t_end = 0, read_s = 0, issue_s = 0;
@@ -576,7 +576,11 @@ int main(int argc, char *argv[])
       if (next_stem) break;
     } /* files */
 
-    if (ing) cli_ingest_discard(ing);
+    if (ing) {
+      int fatal = 0;
+      if (cli_ingest_flush(ing, &fatal) && fatal) return 1;
+      cli_ingest_discard(ing);
+    }
     if (ctx.Nc) rawspec_wait_for_completion(&ctx);
     if (t_start != 0) t_end = now_s();
     if (fdhdrs >= 0) close(fdhdrs);
