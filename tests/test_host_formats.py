@@ -281,7 +281,7 @@ def test_cards_after_end_are_not_searched(L):
 # ---- SIGPROC filterbank headers ---------------------------------------------
 
 def test_fb_value_encoding(L):
-    b = C.create_string_buffer(64)
+    b = C.create_string_buffer(256)
     end = L.fb_buf_write_int(b, -2)
     assert end - C.addressof(b) == 4 and b.raw[:4] == struct.pack("<i", -2)
     end = L.fb_buf_write_double(b, 1420.40575)
