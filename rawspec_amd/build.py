@@ -85,6 +85,29 @@ def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
 
 
 CLI_SOURCES = ["cli_main.c", "cli_ingest.c", "cli_output.c", "cli_net.c"]
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+
+
+def build_fbh5_plugin(out_dir: str, verbose: bool = False):
+    """librawspec_fbh5.so, the FBH5 writer the program loads for -j; built
+    only where the HDF5 headers and libraries (with the high-level library)
+    are found.  Returns its path or None."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5_hl.so"))):
+        if verbose:
+            print(f"no HDF5 under {HDF5_DIR}: the program is built without FBH5 output")
+        return None
+    obj = os.path.join(BUILD, "cli_fbh5_writer.o")
+    out = os.path.join(out_dir, "librawspec_fbh5.so")
+    _run([CC, "-c", *C_FLAGS, f"-I{CLI_SRC}", f"-I{inc}", os.path.join(CLI_SRC, "fbh5_writer.c"),
+          "-o", obj])
+    _run([CC, "-shared", "-o", out + ".tmp", obj, f"-L{libdir}", "-lhdf5_hl", "-lhdf5",
+          f"-Wl,-rpath,{libdir}"])
+    os.replace(out + ".tmp", out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 def build_cli(lib: str = LIB, out: str = CLI, verbose: bool = False) -> str:
@@ -100,8 +123,9 @@ def build_cli(lib: str = LIB, out: str = CLI, verbose: bool = False) -> str:
     tmp = out + ".tmp"
     _run([CC, "-o", tmp, *objs, f"-L{os.path.dirname(lib)}", "-lrawspec",
           "-Wl,-rpath,$ORIGIN/..", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
-          "-lpthread", "-lm"])
+          "-lpthread", "-ldl", "-lm"])
     os.replace(tmp, out)
+    build_fbh5_plugin(os.path.dirname(out), verbose)
     if verbose:
         print(f"built {out}")
     return out

@@ -14,6 +14,7 @@
 #include <stdint.h>
 #include <sys/types.h>
 
+#include "fbh5_writer.h"
 #include "rawspec.h"
 #include "rawspec_fbutils.h"
 
@@ -41,6 +42,11 @@ typedef struct {
   unsigned int Nf;         /* fine channels (Nc * Nt) */
   fb_hdr_t fb_hdr;
   int net;                 /* 1: send UDP packets to fd[0] instead of writing files */
+  int fbh5;                /* 1: FBH5 files (h5, h5_ics) instead of SIGPROC ones */
+  fbh5w_ctx **h5;          /* one per entry of fd */
+  fbh5w_ctx *h5_ics;
+  float *gather;           /* one integration of one antenna (FBH5, split output) */
+  size_t gather_n;         /* floats gather holds */
   volatile unsigned int exit_soon;
 } cli_product;
 
@@ -51,8 +57,14 @@ typedef struct {
 int cli_open_output_file(const char *dest, const char *stem, int output_idx);
 
 /* Opens the product's file(s), one per antenna ("<stem>-antNNN") when split,
- * and writes the filterbank header to each (rawspec_file.c:73-98). */
+ * and writes the filterbank header to each (rawspec_file.c:73-98); with
+ * p->fbh5, FBH5 files "....h5" instead. */
 int cli_open_product_files(cli_product *p, const char *dest, const char *stem, int output_idx);
+/* The incoherent-sum file of a product (stem "<stem>-ics"). */
+int cli_open_ics_file(cli_product *p, const char *dest, const char *ics_stem, int output_idx);
+
+/* Loads the FBH5 plugin (librawspec_fbh5.so next to the program); 0 on success. */
+int cli_fbh5_load(void);
 
 /* Closes the product's files; returns non-zero if a close failed. */
 int cli_close_product_files(cli_product *p);

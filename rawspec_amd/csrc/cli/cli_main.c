@@ -129,9 +129,16 @@ static int reset_fds(cli_product *p, unsigned int n)
 {
   unsigned int a;
   int *fd = (int *)malloc(sizeof(int) * (n ? n : 1));
-  if (!fd) return 1;
+  fbh5w_ctx **h5 = (fbh5w_ctx **)calloc(n ? n : 1, sizeof(fbh5w_ctx *));
+  if (!fd || !h5) {
+    free(fd);
+    free(h5);
+    return 1;
+  }
   free(p->fd);
+  free(p->h5);
   p->fd = fd;
+  p->h5 = h5;
   p->nfd = n ? n : 1;
   for (a = 0; a < p->nfd; a++) p->fd[a] = -1;
   return 0;
@@ -227,16 +234,14 @@ int main(int argc, char *argv[])
     usage(argv0);
     return 1;
   }
-  if (fbh5) {
-    fprintf(stderr, "error: FBH5 output is not available in this build\n");
-    return 1;
-  }
+  if (fbh5 && !net && cli_fbh5_load()) return 1;
   if (ctx.incoherently_sum && per_ant_out) {
     fprintf(stderr, "PLEASE NOTE: Currently, there are potential conflicts in running -i and -S concurrently.\n");
     fprintf(stderr, "PLEASE NOTE: -S (split antennas) is being ignored.\n");
     per_ant_out = 0;
   }
-  if (!net) printf("writing output files in SIGPROC Filterbank format\n");
+  if (!net) printf(fbh5 ? "writing output files in FBH5 format\n"
+                        : "writing output files in SIGPROC Filterbank format\n");
   if (schan != 0 && nchan == 0) {
     fprintf(stderr, "error: nchan must be non-zero if schan is non-zero\n");
     return 1;
@@ -288,6 +293,7 @@ int main(int argc, char *argv[])
     p->Nant = 1;
     p->fd_ics = -1;
     p->net = net;
+    p->fbh5 = fbh5 && !net;
     if (reset_fds(p, 1)) return 1;
   }
   if (net) {
@@ -488,14 +494,8 @@ int main(int argc, char *argv[])
           p->fb_hdr.tsamp = raw_hdr.tbin * ctx.Nts[i] * ctx.Nas[i];
           if (!net) {
             if (!only_ics && cli_open_product_files(p, dest, argv[si], (int)outidx + i)) return 1;
-            if (ctx.incoherently_sum) {
-              p->fd_ics = cli_open_output_file(dest, ics_stem, (int)outidx + i);
-              if (p->fd_ics < 0) {
-                fprintf(stderr, "cannot open output file, giving up\n");
-                return 1;
-              }
-              fb_fd_write_header(p->fd_ics, &p->fb_hdr);
-            }
+            if (ctx.incoherently_sum && cli_open_ics_file(p, dest, ics_stem, (int)outidx + i))
+              return 1;
           }
         }
         if (save_headers) {
@@ -600,7 +600,11 @@ int main(int argc, char *argv[])
   free(ics_stem);
   free(ctx.Aws);
   if (net && prod[0].fd && prod[0].fd[0] >= 0) close(prod[0].fd[0]);
-  for (i = 0; i < (int)ctx.No; i++) free(prod[i].fd);
+  for (i = 0; i < (int)ctx.No; i++) {
+    free(prod[i].fd);
+    free(prod[i].h5);
+    free(prod[i].gather);
+  }
 
   {
     uint64_t tp = 0, tb = 0, tn = 0;

@@ -14,6 +14,7 @@
  * waits for a write only when the next dump of that product is due.
  */
 #define _GNU_SOURCE 1
+#include <dlfcn.h>
 #include <errno.h>
 #include <fcntl.h>
 #include <limits.h>
@@ -24,6 +25,36 @@
 #include <unistd.h>
 
 #include "cli.h"
+
+static fbh5w_open_fn h5_open;
+static fbh5w_write_fn h5_write;
+static fbh5w_close_fn h5_close;
+
+int cli_fbh5_load(void)
+{
+  char exe[PATH_MAX + 1], lib[PATH_MAX + 32];
+  ssize_t n = readlink("/proc/self/exe", exe, PATH_MAX);
+  char *slash;
+  void *h;
+  if (n <= 0) return 1;
+  exe[n] = '\0';
+  slash = strrchr(exe, '/');
+  if (slash) *slash = '\0';
+  snprintf(lib, sizeof lib, "%s/librawspec_fbh5.so", exe);
+  h = dlopen(lib, RTLD_NOW | RTLD_LOCAL);
+  if (!h) {
+    fprintf(stderr, "error: FBH5 output is not available: %s\n", dlerror());
+    return 1;
+  }
+  h5_open = (fbh5w_open_fn)dlsym(h, "fbh5w_open");
+  h5_write = (fbh5w_write_fn)dlsym(h, "fbh5w_write");
+  h5_close = (fbh5w_close_fn)dlsym(h, "fbh5w_close");
+  if (!h5_open || !h5_write || !h5_close) {
+    fprintf(stderr, "error: %s lacks the FBH5 writer\n", lib);
+    return 1;
+  }
+  return 0;
+}
 
 static void output_name(char *out, size_t n, const char *dest, const char *stem,
                         const char *suffix)
@@ -55,6 +86,34 @@ int cli_open_output_file(const char *dest, const char *stem, int output_idx)
   return open_trunc(name);
 }
 
+static fbh5w_ctx *open_h5(cli_product *p, const char *dest, const char *stem, int output_idx)
+{
+  char name[PATH_MAX + 1], suffix[32];
+  snprintf(suffix, sizeof suffix, ".rawspec.%04d.h5", output_idx);
+  output_name(name, sizeof name, dest, stem, suffix);
+  return h5_open(&p->fb_hdr, p->Nds, name, get_librawspec_version(), get_librawspec_version(),
+                 get_cufft_version());
+}
+
+int cli_open_ics_file(cli_product *p, const char *dest, const char *ics_stem, int output_idx)
+{
+  if (p->fbh5) {
+    p->h5_ics = open_h5(p, dest, ics_stem, output_idx);
+    if (!p->h5_ics) {
+      fprintf(stderr, "cannot open output file, giving up\n");
+      return 1;
+    }
+    return 0;
+  }
+  p->fd_ics = cli_open_output_file(dest, ics_stem, output_idx);
+  if (p->fd_ics < 0) {
+    fprintf(stderr, "cannot open output file, giving up\n");
+    return 1;
+  }
+  fb_fd_write_header(p->fd_ics, &p->fb_hdr);
+  return 0;
+}
+
 int cli_open_headers_file(const char *dest, const char *stem)
 {
   char name[PATH_MAX + 1];
@@ -72,6 +131,15 @@ int cli_open_product_files(cli_product *p, const char *dest, const char *stem, i
       snprintf(ant_stem, sizeof ant_stem, "%s-ant%03u", stem, a);
     else
       snprintf(ant_stem, sizeof ant_stem, "%s", stem);
+    if (p->fbh5) {
+      p->h5[a] = open_h5(p, dest, ant_stem, output_idx);
+      if (!p->h5[a]) {
+        fprintf(stderr, "cannot open output file, giving up\n");
+        p->exit_soon = 1;
+        return 1;
+      }
+      continue;
+    }
     p->fd[a] = cli_open_output_file(dest, ant_stem, output_idx);
     if (p->fd[a] < 0) {
       fprintf(stderr, "cannot open output file, giving up\n");
@@ -89,6 +157,13 @@ int cli_close_product_files(cli_product *p)
   unsigned int a;
   const unsigned int n = p->per_ant_out ? p->Nant : 1;
   for (a = 0; a < n && a < p->nfd; a++) {
+    if (p->h5 && p->h5[a]) {
+      if (h5_close(p->h5[a])) {
+        fprintf(stderr, "FBH5-CLOSE-ERROR ant %u\n", a);
+        bad = 1;
+      }
+      p->h5[a] = NULL;
+    }
     if (p->fd[a] >= 0) {
       if (close(p->fd[a]) < 0) {
         fprintf(stderr, "SIGPROC-CLOSE-ERROR ant %u\n", a);
@@ -96,6 +171,13 @@ int cli_close_product_files(cli_product *p)
       }
       p->fd[a] = -1;
     }
+  }
+  if (p->h5_ics) {
+    if (h5_close(p->h5_ics)) {
+      fprintf(stderr, "FBH5-CLOSE-ERROR ics\n");
+      bad = 1;
+    }
+    p->h5_ics = NULL;
   }
   if (p->fd_ics >= 0) {
     if (close(p->fd_ics) < 0) {
@@ -125,12 +207,53 @@ static int write_all(int fd, const void *buf, size_t len)
 static void write_failed(cli_product *p)
 {
   p->exit_soon = 1;
-  fprintf(stderr, "SIGPROC-WRITE-ERROR\n");
+  fprintf(stderr, p->fbh5 ? "FBH5-WRITE-ERROR\n" : "SIGPROC-WRITE-ERROR\n");
+}
+
+/* FBH5 dumps: one antenna writes the whole buffer ([Nds][nifs][nchans]);
+ * split per antenna, each spectrum's nifs runs of that antenna are gathered
+ * into one integration.  (The reference passes each pol's run on its own,
+ * which is less than one integration when nifs > 1.) */
+static void fbh5_dump(cli_product *p)
+{
+  if (p->h_pwrbuf && p->Nant == 1 && p->h5 && p->h5[0]) {
+    if (h5_write(p->h5[0], p->h_pwrbuf, p->h_pwrbuf_size)) write_failed(p);
+  }
+  if (p->h_pwrbuf && p->Nant > 1 && p->per_ant_out && p->h5) {
+    const size_t spectrum = p->h_pwrbuf_size / ((size_t)p->Nds * sizeof(float));
+    const size_t pol = spectrum / (size_t)p->fb_hdr.nifs;
+    const size_t ant = pol / p->Nant;
+    const size_t need = (size_t)p->fb_hdr.nifs * ant;
+    size_t k, j, a;
+    if (p->gather_n < need) {
+      free(p->gather);
+      p->gather = (float *)malloc(sizeof(float) * need);
+      p->gather_n = p->gather ? need : 0;
+    }
+    if (!p->gather) write_failed(p);
+    for (k = 0; k < p->Nds && p->gather; k++)
+      for (a = 0; a < p->Nant; a++) {
+        if (!p->h5[a]) break;
+        for (j = 0; j < (size_t)p->fb_hdr.nifs; j++)
+          memcpy(p->gather + j * ant, p->h_pwrbuf + a * ant + j * pol + k * spectrum,
+                 ant * sizeof(float));
+        if (h5_write(p->h5[a], p->gather, (size_t)p->fb_hdr.nifs * ant * sizeof(float)))
+          write_failed(p);
+      }
+  }
+  if (p->h5_ics && p->h_icsbuf) {
+    if (h5_write(p->h5_ics, p->h_icsbuf, p->h_pwrbuf_size / p->Nant)) write_failed(p);
+  }
 }
 
 static void *file_writer(void *arg)
 {
   cli_product *p = (cli_product *)arg;
+  if (p->fbh5) {
+    fbh5_dump(p);
+    p->total_spectra += p->Nds;
+    return NULL;
+  }
   if (p->h_pwrbuf && p->Nant == 1 && p->fd[0] >= 0) {
     if (write_all(p->fd[0], p->h_pwrbuf, p->h_pwrbuf_size)) write_failed(p);
   }

@@ -17,8 +17,9 @@ import numpy as np
 import pytest
 
 import oracle.rawstem as RS
-from tests.cli_cases import (StemSpec, check_fil, check_split, expected, header_fields, run_cli,
-                             write_stem)
+from tests import fbh5_read as F
+from tests.cli_cases import (CLI, StemSpec, check_fil, check_split, expected, header_fields,
+                             run_cli, write_stem)
 
 pytestmark = pytest.mark.gpu
 
@@ -252,3 +253,49 @@ def test_network_output(tmp_path):
         assert spectra.shape == want.shape, (i, spectra.shape, want.shape, per_pkt)
         den = np.maximum(want, want.mean(axis=1, keepdims=True))
         assert float((np.abs(spectra - want) / den).max()) <= 1e-5
+
+
+FBH5_OK = os.path.exists(os.path.join(os.path.dirname(CLI), "librawspec_fbh5.so")) and F.available()
+
+
+def _fil_payload(path):
+    raw = open(path, "rb").read()
+    n = raw.index(b"HEADER_END") + len(b"HEADER_END")
+    return np.frombuffer(raw[n:], dtype=np.float32)
+
+
+@pytest.mark.skipif(not FBH5_OK, reason="FBH5 plugin or h5dump not available")
+@pytest.mark.parametrize("case", ["one", "split", "ics"])
+def test_fbh5_output(tmp_path, case):
+    """-j writes FBH5 files holding the same spectra as the SIGPROC files of
+    the same run, with the filterbank header as data set attributes."""
+    nants = 1 if case == "one" else 2
+    spec = StemSpec(obsnchan=8, nants=nants, ntpb=4096, files=[[0, 1, 2, 3]])
+    stem = str(tmp_path / "h")
+    paths = write_stem(spec, stem)
+    fil, h5 = tmp_path / "fil", tmp_path / "h5"
+    fil.mkdir()
+    h5.mkdir()
+    args = ["-f", "1024,8", "-t", "2,512", "-p", "-4,1"]
+    if case == "ics":
+        args += ["-i", "1.0,0.5"]
+    run_cli([*args, "-d", str(fil), stem])
+    r = run_cli([*args, "-j", "-d", str(h5), stem])
+    assert "FBH5" in r.stdout
+    _, _, cfg, _ = expected(spec, Nts=[1024, 8], Nas=[2, 512], Npolout=[-4, 1])
+    for i in range(2):
+        f = header_fields(spec, cfg, i, rawdatafile=os.path.basename(paths[0]))
+        names = ({"one": [dict()], "split": [dict(ant=a) for a in range(nants)],
+                  "ics": [dict(ics=True)]})[case]
+        for kw in names:
+            fa, da, info, data = F.read(RS.output_name(stem, i, dest=str(h5), **kw)[:-4] + ".h5")
+            want = _fil_payload(RS.output_name(stem, i, dest=str(fil), **kw))
+            assert fa["CLASS"] == "FILTERBANK" and fa["VERSION"] == "2.0"
+            assert info["shape"][1:] == (f["nifs"], f["nchans"])
+            assert info["chunk"] == (cfg.Nds[i], 1, cfg.Nts[i])
+            assert np.array_equal(data.reshape(-1), want)
+            for k in ("nchans", "nfpc", "nifs", "nbits", "telescope_id", "machine_id"):
+                assert da[k] == f[k], k
+            for k in ("fch1", "foff", "tstart", "tsamp", "src_raj", "src_dej"):
+                assert da[k] == f[k], k
+            assert da["source_name"] == f["source_name"]
