@@ -435,23 +435,30 @@ static int rs_dump(rawspec_context *ctx, rs_gpu_ctx *g, unsigned int i)
   return 0;
 }
 
+/* The products' kernels run on the device layer's lanes (four-step and small
+ * products side by side); every dump waits for its product's lane and runs,
+ * in product order, on the context stream, which waits for both lanes before
+ * the call returns. */
 static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
 {
   unsigned int i;
+  int last[MAX_OUTPUTS];
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
   g->inbuf_count++;
+  if (rsd_lanes_begin(g->dev)) return 1;
   for (i = 0; i < ctx->No; i++) {
     int first = ((g->inbuf_count - 1) % g->Nis[i]) == 0;
-    int last = (g->inbuf_count % g->Nis[i]) == 0;
-    if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first, last)) {
+    last[i] = (g->inbuf_count % g->Nis[i]) == 0;
+    if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first, last[i])) {
       fprintf(stderr, "librawspec: processing output product %u failed\n", i);
       fflush(stderr);
       return 1;
     }
-    if (last && rs_dump(ctx, g, i)) return 1;
   }
-  return 0;
+  for (i = 0; i < ctx->No; i++)
+    if (last[i] && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i))) return 1;
+  return rsd_lane_end(g->dev, -1);
 }
 
 /* nbuf consecutive buffers: per product, the run is cut at integration
@@ -464,6 +471,7 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
   unsigned int i;
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
+  if (rsd_lanes_begin(g->dev)) return 1;
   for (i = 0; i < ctx->No; i++) {
     unsigned int k = 0;
     while (k < nbuf) {
@@ -476,12 +484,15 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
         fflush(stderr);
         return 1;
       }
-      if (last && rs_dump(ctx, g, i)) return 1;
+      /* the dump reads the integration buffer the lane's next piece rewrites */
+      if (last && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i) ||
+                   (k + len < nbuf && rsd_lane_resume(g->dev, (int)i))))
+        return 1;
       k += len;
     }
   }
   g->inbuf_count += nbuf;
-  return 0;
+  return rsd_lane_end(g->dev, -1);
 }
 
 int rawspec_start_processing(rawspec_context *ctx, int fft_dir)

@@ -70,6 +70,17 @@ int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first
 int  rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n, int inverse,
                            int first, int last);
 /* Enqueues the incoherent sum of product i's integration buffer. */
+/* Products run on two lanes (streams) forked from the context stream:
+ * four-step products on one, small products on the other.  rsd_lanes_begin
+ * makes both lanes wait for everything enqueued on the context stream;
+ * rsd_lane_end makes the context stream wait for product i's lane (i < 0:
+ * both lanes); rsd_lane_resume makes product i's lane wait for the context
+ * stream (after a dump that must precede the lane's next writes).  All are
+ * no-ops while kernel timing is on (then everything runs on the context
+ * stream). */
+int  rsd_lanes_begin(rsd_ctx *d);
+int  rsd_lane_end(rsd_ctx *d, int i);
+int  rsd_lane_resume(rsd_ctx *d, int i);
 int  rsd_run_ics(rsd_ctx *d, int i);
 /* Enqueues the device -> host copy of product i's dump (and ICS dump when
  * h_ics is not NULL). */

@@ -95,6 +95,13 @@ std::vector<float2> twiddles(unsigned long long n, unsigned long long cnt, unsig
 struct rsd_ctx {
   int dev;
   hipStream_t stream;
+  // Products run on two lanes forked from `stream` and joined back into it:
+  // four-step products (which share d_work) on lane 0, small products (which
+  // share d_part) on lane 1, so the vector-bound small transforms overlap
+  // the memory-bound four-step passes.  Dumps stay on `stream`.
+  hipStream_t lane[2];
+  hipEvent_t ev_main, ev_lane[2];
+  bool lanes;
   rsd_config cfg;
   std::vector<float> weights;
   unsigned long long sample_bytes;     // Np * 2 * Nbps/8
@@ -129,11 +136,23 @@ static hipEvent_t ev_take(rsd_ctx *d) {
   return e;
 }
 
+// Lanes are used unless kernel timing is on: timed kernels run one after
+// another on the context stream, so each event pair brackets one kernel.
+static bool use_lanes(const rsd_ctx *d) { return d->lanes && !d->prof; }
+
+static hipStream_t prod_stream(const rsd_ctx *d, const ProdPlan &pp) {
+  return use_lanes(d) ? d->lane[pp.small ? 1 : 0] : d->stream;
+}
+
 // Brackets one kernel launch with events when timing is on.
 struct ProfScope {
   rsd_ctx *d;
   long idx;
-  ProfScope(rsd_ctx *dd, int product, const char *kname, unsigned n) : d(dd), idx(-1) {
+  hipStream_t s;
+  ProfScope(rsd_ctx *dd, int product, const char *kname, unsigned n)
+      : ProfScope(dd, product, kname, n, dd->stream) {}
+  ProfScope(rsd_ctx *dd, int product, const char *kname, unsigned n, hipStream_t st)
+      : d(dd), idx(-1), s(st) {
     if (!d->prof) return;
     hipEvent_t a = ev_take(d), b = ev_take(d);
     if (!a || !b) return;
@@ -141,10 +160,10 @@ struct ProfScope {
     snprintf(nm, sizeof nm, "p%d.%s.N%u", product, kname, n);
     d->recs.push_back({nm, a, b});
     idx = (long)d->recs.size() - 1;
-    (void)hipEventRecord(a, d->stream);
+    (void)hipEventRecord(a, s);
   }
   ~ProfScope() {
-    if (idx >= 0) (void)hipEventRecord(d->recs[idx].b, d->stream);
+    if (idx >= 0) (void)hipEventRecord(d->recs[idx].b, s);
   }
 };
 
@@ -252,6 +271,14 @@ void rsd_destroy(rsd_ctx *d) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : d->pool) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; ++k) {
+    if (d->lane[k]) {
+      (void)hipStreamSynchronize(d->lane[k]);
+      (void)hipStreamDestroy(d->lane[k]);
+    }
+    if (d->ev_lane[k]) (void)hipEventDestroy(d->ev_lane[k]);
+  }
+  if (d->ev_main) (void)hipEventDestroy(d->ev_main);
   if (d->stream) (void)hipStreamDestroy(d->stream);
   delete d;
 }
@@ -376,6 +403,12 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
   if (alloc_zero((void **)&d->d_part, d->part_len * sizeof(float))) return 1;
   if (alloc_zero((void **)&d->d_work, d->work_len * sizeof(float2))) return 1;
   RSD_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  RSD_CHECK(hipEventCreateWithFlags(&d->ev_main, hipEventDisableTiming));
+  for (int k = 0; k < 2; ++k) {
+    RSD_CHECK(hipStreamCreateWithFlags(&d->lane[k], hipStreamNonBlocking));
+    RSD_CHECK(hipEventCreateWithFlags(&d->ev_lane[k], hipEventDisableTiming));
+  }
+  d->lanes = true;
   return 0;
 }
 
@@ -383,6 +416,9 @@ static rsd_ctx *new_ctx() {
   rsd_ctx *d = new rsd_ctx();
   memset(d->pp, 0, sizeof(d->pp));
   d->stream = nullptr;
+  d->lane[0] = d->lane[1] = nullptr;
+  d->ev_main = d->ev_lane[0] = d->ev_lane[1] = nullptr;
+  d->lanes = false;
   d->d_in = d->d_stage = nullptr;
   d->d_part = nullptr;
   d->d_work = d->d_tw = nullptr;
@@ -454,6 +490,7 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
   const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
   const float scale = 1.0f / (full * full);
   if (pp.small) {
+    const hipStream_t s = prod_stream(d, pp);
     rsk::SmallArgs a;
     a.in = in;
     a.chan_stride = d->chan_stride;
@@ -483,16 +520,16 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
       a.direct = pp.m_G == 1;
       const unsigned long long items = (unsigned long long)c.Nc * pp.Ng * pp.m_G;
       if (items > 0x7fffffffull) return 1;
-      ProfScope ps(d, i, "k_small_mfma", pp.p.Nt);
+      ProfScope ps(d, i, "k_small_mfma", pp.p.Nt, s);
       if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
-                                 (unsigned)items, d->stream))
+                                 (unsigned)items, s))
         return 1;
     } else {
       const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
       if (nb > 0x7fffffffull) return 1;
-      ProfScope ps(d, i, "k_small", pp.p.Nt);
+      ProfScope ps(d, i, "k_small", pp.p.Nt, s);
       if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
-                            d->stream))
+                            s))
         return 1;
     }
     if (!a.direct) {
@@ -506,8 +543,8 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
       r.N = pp.p.Nt;
       r.scale = scale;
       r.first = first;
-      ProfScope ps(d, i, "k_reduce", pp.p.Nt);
-      if (rsk::launch_reduce(r, d->stream)) return 1;
+      ProfScope ps(d, i, "k_reduce", pp.p.Nt, s);
+      if (rsk::launch_reduce(r, s)) return 1;
     }
     return 0;
   }
@@ -523,6 +560,7 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
   ProdPlan &pp = d->pp[i];
   const rsd_config &c = d->cfg;
   const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
+  const hipStream_t s = prod_stream(d, pp);
   rsk::FsArgs a;
   a.chan_stride = d->chan_stride;
   a.Nc = c.Nc;
@@ -565,15 +603,15 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
       a.c0 = c0;
       a.nch = c.Nc - c0 < chc ? c.Nc - c0 : chc;
       {
-        ProfScope ps(d, i, "k_fs_a", pp.p.Nt);
-        if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, d->stream)) return 1;
+        ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
+        if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, s)) return 1;
       }
       if (pp.fs_mfma) {
-        ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt);
-        if (rsk::launch_fs_b_mfma((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+        ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt, s);
+        if (rsk::launch_fs_b_mfma((int)c.Np, pp.p.npolout, a, s)) return 1;
       } else {
-        ProfScope ps(d, i, "k_fs_b", pp.p.Nt);
-        if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, d->stream)) return 1;
+        ProfScope ps(d, i, "k_fs_b", pp.p.Nt, s);
+        if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, s)) return 1;
       }
     }
     j += kb;
@@ -597,6 +635,33 @@ int rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n
   const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
   return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, isign, gam,
                       1.0f / (full * full), first, last);
+}
+
+int rsd_lanes_begin(rsd_ctx *d) {
+  if (!use_lanes(d)) return 0;
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipEventRecord(d->ev_main, d->stream));
+  for (int k = 0; k < 2; ++k) RSD_CHECK(hipStreamWaitEvent(d->lane[k], d->ev_main, 0));
+  return 0;
+}
+
+int rsd_lane_end(rsd_ctx *d, int i) {
+  if (!use_lanes(d)) return 0;
+  RSD_CHECK(hipSetDevice(d->dev));
+  for (int k = 0; k < 2; ++k) {
+    if (i >= 0 && k != (d->pp[i].small ? 1 : 0)) continue;
+    RSD_CHECK(hipEventRecord(d->ev_lane[k], d->lane[k]));
+    RSD_CHECK(hipStreamWaitEvent(d->stream, d->ev_lane[k], 0));
+  }
+  return 0;
+}
+
+int rsd_lane_resume(rsd_ctx *d, int i) {
+  if (!use_lanes(d)) return 0;
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipEventRecord(d->ev_main, d->stream));
+  RSD_CHECK(hipStreamWaitEvent(d->lane[d->pp[i].small ? 1 : 0], d->ev_main, 0));
+  return 0;
 }
 
 int rsd_run_ics(rsd_ctx *d, int i) {
