@@ -18,7 +18,11 @@ timed: "plain", one buffer per rawspec_amd_start_processing_device() call
 vector units; "served", batches of 8 consecutive buffers per
 rawspec_amd_start_processing_device_batch() call; and "served_mfma", the
 same on the matrix-core engine.  `value` is the faster served path (both
-keep the plain path's error bar).
+keep the plain path's error bar).  The library runs this context's
+four-step and small products on two streams that overlap (rawspec_amd_set_
+lanes; --lanes off for one stream); the per-kernel times, and the roofline
+of the dominant kernel, come from a pass with HIP events around every
+kernel, in which the products run one after another.
 
 Also measured, reported under "configs": the other single-stream workloads
 (config 1: 8 ch, 4 MiB buffers, Nfft 1024; config 2: config 3's geometry
@@ -318,6 +322,9 @@ def main(argv=None):
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
     ap.add_argument("--legs", default="plain,served,served_mfma",
                     help="paths to time (comma-separated subset of the default)")
+    ap.add_argument("--lanes", choices=["auto", "on", "off"], default="auto",
+                    help="two-lane execution of the headline context (auto: the library's "
+                         "choice); per-kernel times always come from a pass without lanes")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 1 GiB device copy 4 times (calibrates the HBM "
                          "byte counters of a rocprofv3 --pmc run of this command)")
@@ -341,8 +348,12 @@ def main(argv=None):
     buf_bytes = sp.device_input_size()
     assert buf_bytes == c.Nc * c.Nb * c.Ntpb * c.Np * 2
     sp.set_device_output(True)
+    if a.lanes != "auto":
+        assert sp.set_lanes(a.lanes == "on") == 0
+    lanes = sp.get_lanes()
     ring = gen_ring(torch, sp, W, a.ring)
-    log(f"bench: ring of {len(ring)} device buffers x {buf_bytes / 2**20:.0f} MiB ready")
+    log(f"bench: ring of {len(ring)} device buffers x {buf_bytes / 2**20:.0f} MiB ready; "
+        f"lanes {'on' if lanes else 'off'}")
 
     def measure(batch, mfma):
         """Throughput of one leg, then its per-kernel HIP-event timing."""
@@ -474,7 +485,9 @@ def main(argv=None):
                    "path": f"{head_name}: batches of {a.batch} consecutive buffers per call, "
                            f"{'matrix-core' if head_name == 'served_mfma' else 'FP32 vector'} engine",
                    "step_bytes": buf_bytes, "ring_buffers": a.ring, "input": "HBM-resident",
-                   "dumps": "device-resident"},
+                   "dumps": "device-resident",
+                   "lanes": "four-step and small products on two streams" if lanes else "off",
+                   "kernel_times": "HIP events from a pass with the products in sequence"},
         "paths": {k: {"GBps": round(v["GBps"], 2), "ms_per_step": round(v["ms_per_step"], 4),
                       "vs_plain": round(v["GBps"] / legs["plain"]["GBps"], 3)}
                   for k, v in legs.items()},

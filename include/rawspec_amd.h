@@ -75,8 +75,22 @@ void *rawspec_amd_stream(rawspec_context *ctx);
 int rawspec_amd_set_mfma(rawspec_context *ctx, int enable);
 int rawspec_amd_get_mfma(rawspec_context *ctx);
 
+/* Two-lane execution: the four-step products' kernels run on one stream and
+ * the small products' on another, both forked from and joined back into the
+ * context's stream within each start call, so vector-bound small transforms
+ * overlap memory-bound four-step passes.  Dumps and callbacks keep their
+ * order on the context's stream.  rawspec_initialize() turns it on when the
+ * context has both kinds of products and buffers of at least 64 MiB; a
+ * process running many contexts at once (more streams than hardware queues)
+ * may do better with it off.  Changing it between calls is allowed.
+ * rawspec_amd_set_lanes returns 0 on success; rawspec_amd_get_lanes returns
+ * 1 or 0 (-1 before initialisation). */
+int rawspec_amd_set_lanes(rawspec_context *ctx, int enable);
+int rawspec_amd_get_lanes(rawspec_context *ctx);
+
 /* Kernel timing.  With enable != 0 every kernel the context launches is
- * bracketed by HIP events on its stream.  rawspec_amd_profile_report()
+ * bracketed by HIP events on its stream (the lanes are not used meanwhile,
+ * so each pair times one kernel running alone).  rawspec_amd_profile_report()
  * waits for the stream, writes one line per kernel, "p<product>.<kernel>.N<Nt>
  * <launches> <total milliseconds>", into buf (NUL-terminated) and clears the
  * records.  Returns the number of characters written, -1 on error. */

@@ -623,6 +623,21 @@ int rawspec_amd_set_mfma(rawspec_context *ctx, int enable)
   return rsd_set_mfma(g->dev, enable);
 }
 
+/* Every start call joins the lanes back into the context stream before it
+ * returns, so the setting may change between any two calls. */
+int rawspec_amd_set_lanes(rawspec_context *ctx, int enable)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  return rsd_set_lanes(g->dev, enable);
+}
+
+int rawspec_amd_get_lanes(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = (rs_gpu_ctx *)ctx->gpu_ctx;
+  return g && g->dev ? rsd_get_lanes(g->dev) : -1;
+}
+
 int rawspec_amd_get_mfma(rawspec_context *ctx)
 {
   rs_gpu_ctx *g = rs_get(ctx);

@@ -75,9 +75,9 @@ int  rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned 
  * makes both lanes wait for everything enqueued on the context stream;
  * rsd_lane_end makes the context stream wait for product i's lane (i < 0:
  * both lanes); rsd_lane_resume makes product i's lane wait for the context
- * stream (after a dump that must precede the lane's next writes).  All are
- * no-ops while kernel timing is on (then everything runs on the context
- * stream). */
+ * stream (after a dump that must precede the lane's next writes).  With
+ * lanes off, or while kernel timing is on, all are no-ops and everything
+ * runs on the context stream. */
 int  rsd_lanes_begin(rsd_ctx *d);
 int  rsd_lane_end(rsd_ctx *d, int i);
 int  rsd_lane_resume(rsd_ctx *d, int i);
@@ -113,6 +113,9 @@ int rsd_profile_report(rsd_ctx *d, char *buf, size_t len);
  * (split-FP16 MFMA) where a kernel exists for the shape.  A product switches
  * at the start of its next integration. */
 int rsd_set_mfma(rsd_ctx *d, int enable);
+/* Two-lane execution on (default where it pays) or off. */
+int rsd_set_lanes(rsd_ctx *d, int enable);
+int rsd_get_lanes(rsd_ctx *d);
 int rsd_get_mfma(rsd_ctx *d);
 
 #ifdef __cplusplus

@@ -75,6 +75,8 @@ struct ProdPlan {
 #define RSD_FS_WORK_MB 1536
 #endif
 constexpr size_t kFsWorkBytes = (size_t)RSD_FS_WORK_MB << 20;
+// smallest input buffer for which a context runs its products on two lanes
+constexpr size_t kLaneMinBytes = (size_t)64 << 20;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -137,7 +139,8 @@ static hipEvent_t ev_take(rsd_ctx *d) {
 }
 
 // Lanes are used unless kernel timing is on: timed kernels run one after
-// another on the context stream, so each event pair brackets one kernel.
+// another on the context stream, so each event pair brackets one kernel
+// running alone.
 static bool use_lanes(const rsd_ctx *d) { return d->lanes && !d->prof; }
 
 static hipStream_t prod_stream(const rsd_ctx *d, const ProdPlan &pp) {
@@ -408,7 +411,12 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
     RSD_CHECK(hipStreamCreateWithFlags(&d->lane[k], hipStreamNonBlocking));
     RSD_CHECK(hipEventCreateWithFlags(&d->ev_lane[k], hipEventDisableTiming));
   }
-  d->lanes = true;
+  // Lanes pay when a buffer is large enough for the overlap to outweigh the
+  // extra stream synchronisation (two event waits per lane per call) and
+  // when both lanes have work.
+  bool fs = false, sm = false;
+  for (unsigned i = 0; i < cfg->No; ++i) (d->pp[i].small ? sm : fs) = true;
+  d->lanes = fs && sm && d->in_bytes >= kLaneMinBytes;
   return 0;
 }
 
@@ -732,6 +740,14 @@ int rsd_set_mfma(rsd_ctx *d, int enable) {
 }
 
 int rsd_get_mfma(rsd_ctx *d) { return d->mfma; }
+
+int rsd_set_lanes(rsd_ctx *d, int enable) {
+  if (enable && !(d->lane[0] && d->lane[1])) return 1;
+  d->lanes = enable != 0;
+  return 0;
+}
+
+int rsd_get_lanes(rsd_ctx *d) { return d->lanes ? 1 : 0; }
 
 int rsd_profile_report(rsd_ctx *d, char *buf, size_t len) {
   RSD_CHECK(hipSetDevice(d->dev));

@@ -189,6 +189,10 @@ class _Stream:
             raise RuntimeError("rawspec_amd_set_device_output failed")
         if self.sp.set_mfma(srv.mfma):
             raise RuntimeError("rawspec_amd_set_mfma failed")
+        # one stream per context: dozens of concurrent contexts with three
+        # streams each would share the few hardware queues badly
+        if self.sp.set_lanes(srv.lanes):
+            raise RuntimeError("rawspec_amd_set_lanes failed")
         assert self.sp.device_input_size() == plan_.buf_bytes
         self.hs = torch.cuda.ExternalStream(self.sp.stream(), device=f"cuda:{srv.gpu}")
         self.ring = srv.make_ring(req, plan_)
@@ -256,10 +260,12 @@ class Server:
     """
 
     def __init__(self, hbm_budget=200e9, host_budget=96 * GiB, max_active=32, depth=2, ring=2,
-                 gpu=0, record_dumps=False, poll_s=20e-6, stream_cls=None, batch=1, mfma=False):
+                 gpu=0, record_dumps=False, poll_s=20e-6, stream_cls=None, batch=1, mfma=False,
+                 lanes=False):
         self.hbm_budget, self.host_budget = hbm_budget, host_budget
         self.max_active, self.depth, self.nring = max_active, depth, ring
         self.batch, self.mfma = max(1, int(batch)), bool(mfma)
+        self.lanes = bool(lanes)
         self.gpu, self.record_dumps, self.poll_s = gpu, record_dumps, poll_s
         self.stream_cls = stream_cls or _Stream
         self._gen = None

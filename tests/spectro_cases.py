@@ -194,13 +194,15 @@ def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0):
 
 def run_device_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, batches=None,
                     fft_dir=1, conj=0, Nant=1, ics=False, Aws=None, seed=1, sigma=8.0,
-                    tone=None, mfma=False):
+                    tone=None, mfma=False, lanes=None, events=None):
     """Runs nbuf device-resident buffers through rawspec_amd_start_processing_device
     (batches None) or rawspec_amd_start_processing_device_batch (batches: list
     of batch sizes summing to nbuf), dumps copied to the host, and the same
     buffers through the oracle.  Returns (results, cfg) with results as in
     run_case: (buffer, product, gpu, ref, gpu_ics, ref_ics, fp32_ref) per dump,
-    matched per product in dump order."""
+    matched per product in dump order.  lanes: force two-lane execution on or
+    off (None: the library's choice); events: a list that receives every
+    (product, callback type) in call order."""
     import torch
     import rawspec_amd as R
 
@@ -209,6 +211,8 @@ def run_device_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, b
     got = {i: [] for i in range(No)}
 
     def on_dump(sp, product, cbtype):
+        if events is not None:
+            events.append((product, cbtype))
         if cbtype == R.RAWSPEC_CALLBACK_POST_DUMP:
             got[product].append((sp.pwrbuf(product).copy(),
                                  sp.icsbuf(product).copy() if ics else None))
@@ -218,6 +222,8 @@ def run_device_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, b
                         Aws=Aws, callback=on_dump)
     if mfma:
         assert sp.set_mfma(True) == 0
+    if lanes is not None:
+        assert sp.set_lanes(lanes) == 0 and sp.get_lanes() == int(bool(lanes))
     cfg = O.make_cfg(No=No, Np=Np, Nant=Nant, Nc=Nc, Ntpb=Ntpb, Nbps=Nbps, Npolout=Npolout,
                      Nts=Nts, Nas=Nas, input_conjugated=conj,
                      incoherently_sum=1 if ics else 0, Naws=len(Aws) if Aws is not None else 0)
