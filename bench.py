@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """RAW ingest throughput of librawspec (gfx950 HIP spectrometer core) on one GPU.
 
-Headline workload (config 3 of the benchmark set): 1 antenna, 64 coarse
+Headline workload ("stokes_64ch_int8_3products"): 1 antenna, 64 coarse
 channels, 2 pols, complex int8 samples round(N(0, 8^2)) clipped to +-127
 with a complex tone of amplitude 20 at fine bin 300/1024 of channel 3
 (pol 0); blocks of 524288 time samples (128 MiB), so Nb = 2 blocks (256 MiB)
@@ -24,10 +24,10 @@ lanes; --lanes off for one stream); the per-kernel times, and the roofline
 of the dominant kernel, come from a pass with HIP events around every
 kernel, in which the products run one after another.
 
-Also measured, reported under "configs": the other single-stream workloads
-(config 1: 8 ch, 4 MiB buffers, Nfft 1024; config 2: config 3's geometry
-with pols 1 and pols 4; config 4: 16 antennas x 16 channels of complex
-int16, four products with the incoherent sum) and, under "serving", the
+Also measured, reported under "workloads": the other single-stream
+workloads (8 channels with 4 MiB buffers and Nfft 1024; the headline geometry
+with pols 1 and pols 4; 16 antennas x 16 channels of complex int16 with four
+products and the incoherent sum) and, under "serving", the
 serving loop (32 Poisson-arriving streams of random shapes, concurrent vs
 one at a time); under "file_ingest", the rawspec program reading a GUPPI RAW
 file of the headline geometry from the page cache and writing filterbank
@@ -53,19 +53,19 @@ if ROOT not in sys.path:
 METRIC = "RAW ingest GB/s on 1 GPU, MFMA/served vs plain FP32, ≤1e-3 rel err; p99 latency"
 HBM_PEAK_GBS = 8000.0
 
-C3 = dict(name="config3", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128, Nts=[1 << 20, 8, 1024],
+C3 = dict(name="stokes_64ch_int8_3products", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128, Nts=[1 << 20, 8, 1024],
           Nas=[51, 128, 3072], Npolout=[-4, -4, -4], sigma=8.0, tone=(3, 20.0, 300, 1024),
           seed=3, Nant=1)
 OTHER = [
-    dict(name="config1", Nc=8, Np=2, Nbps=8, Ntpb=131072, nblocks=4, Nts=[1024], Nas=[16],
+    dict(name="power_8ch_int8_nfft1024", Nc=8, Np=2, Nbps=8, Ntpb=131072, nblocks=4, Nts=[1024], Nas=[16],
          Npolout=[1], sigma=8.0, tone=(3, 20.0, 300, 1024), seed=1, Nant=1),
-    dict(name="config2_pols1", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
+    dict(name="power_64ch_int8_3products", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
          Nts=[1 << 20, 8, 1024], Nas=[51, 128, 3072], Npolout=[1, 1, 1], sigma=8.0, tone=None,
          seed=2, Nant=1),
-    dict(name="config2_pols4", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
+    dict(name="crosspol_64ch_int8_3products", Nc=64, Np=2, Nbps=8, Ntpb=524288, nblocks=128,
          Nts=[1 << 20, 8, 1024], Nas=[51, 128, 3072], Npolout=[4, 4, 4], sigma=8.0, tone=None,
          seed=2, Nant=1),
-    dict(name="config4", Nc=256, Np=2, Nbps=16, Ntpb=262144, nblocks=64,
+    dict(name="power_16ant_16ch_int16_4products_ics", Nc=256, Np=2, Nbps=16, Ntpb=262144, nblocks=64,
          Nts=[64, 4096, 131072, 8], Nas=[2048, 32, 4, 16384], Npolout=[1, 1, 1, 1],
          sigma=1000.0, tone=None, seed=4, Nant=16),
 ]
@@ -135,7 +135,7 @@ def measured_traffic(kernel, batch):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by
     the counter's reading for a 1 GiB copy), or None when none matches."""
-    path = os.path.join(ROOT, "profiles", "config3_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", "stokes_64ch_int8_3products_hbm_traffic.json")
     try:
         rec = json.load(open(path))
     except (OSError, ValueError):
@@ -341,7 +341,7 @@ def main(argv=None):
     import rawspec_amd as R
 
     W = C3
-    log(f"bench: workload config3 Nc={W['Nc']} Ntpb={W['Ntpb']} Nts={W['Nts']} Nas={W['Nas']} "
+    log(f"bench: workload {W['name']} Nc={W['Nc']} Ntpb={W['Ntpb']} Nts={W['Nts']} Nas={W['Nas']} "
         f"pols={W['Npolout']}")
     sp = make_sp(R, W)
     c = sp.ctx
@@ -479,7 +479,7 @@ def main(argv=None):
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "config3: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
+        "config": {"workload": "stokes_64ch_int8_3products: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
                                "blocks of 524288 samples, Nb=2 (256 MiB/step), "
                                "Nfft {1048576,8,1024} x Nint {51,128,3072}, pols -4",
                    "path": f"{head_name}: batches of {a.batch} consecutive buffers per call, "
@@ -498,7 +498,7 @@ def main(argv=None):
         "latency_ms": {"plain": dict(lat_plain, mode="submit->complete, one buffer"),
                        "served": dict(lat_served, mode=f"submit->complete, one batch of {a.batch}")},
         "cpu_baseline": cpu,
-        "configs": configs,
+        "workloads": configs,
         "file_ingest": ingest,
         "serving": serve,
     }

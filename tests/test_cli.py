@@ -55,7 +55,7 @@ def _check_products(stem, dest, bfname, spec, pw, cfg, Npolout, schan=0):
     return worst
 
 
-def test_config1_stem(tmp_path):
+def test_8ch_int8_stem_nfft1024(tmp_path):
     """8 coarse channels x 2 pols int8, 4 blocks of 131072 samples, tone at
     bin 300 of channel 3; Nfft 1024, Nint 16, Stokes I."""
     spec = StemSpec(obsnchan=8, ntpb=131072, files=[[0, 1, 2, 3]])
@@ -64,7 +64,7 @@ def test_config1_stem(tmp_path):
     assert reason == "end" and len(pw[0]) == 4 and cfg.Nds[0] == 8
     worst = _check_products(stem, dest, bf, spec, pw, cfg, [1])
     assert "output product 0: 32 spectra" in r.stdout
-    print(f"config-1 stem: worst rel err {worst:.2e}")
+    print(f"8-channel stem: worst rel err {worst:.2e}")
 
 
 def test_gaps_files_and_products(tmp_path):

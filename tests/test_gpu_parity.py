@@ -19,7 +19,7 @@ def _run(tol=PLAIN_TOL, **kw):
     return res, cfg, ev
 
 
-def test_config1_small_plain():
+def test_8ch_int8_nfft1024_plain():
     # 8 ch, int8 sigma 8 + tone (ch 3, bin 300/1024), 4 blocks of 131072,
     # Nfft 1024, Nint 16, pols 1, forward (the CLI direction)
     res, cfg, ev = _run(Nc=8, Ntpb=131072, Nts=[1024], Nas=[16], nbuf=4,
@@ -62,7 +62,7 @@ def test_int16_ics_multi_antenna():
          conj=1)
 
 
-def test_config4_product_set_sixteen_antennas():
+def test_sixteen_antennas_int16_four_products_ics():
     """The batched multi-antenna product set at full block length: 16
     antennas x 4 coarse channels each (the full geometry has 16 each; the
     channels are independent), 2 pols of complex int16 ~ round(N(0, 1000^2)),
