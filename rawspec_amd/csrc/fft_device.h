@@ -124,13 +124,29 @@ constexpr int lds_len() { return RSK_LDS_PAD > 0 ? N + (N >> RSK_LDS_PAD) : N; }
 template <int N, int NP>
 constexpr int team_stride() { return (RSK_SEQ_POL ? 1 : NP) * lds_len<N>() + RSK_TEAM_PAD; }
 
+// Synchronises a team between the write and the read half of an LDS
+// exchange.  A team whose threads are consecutive lanes of one wavefront
+// (WAVE) needs no workgroup barrier: a wavefront's LDS instructions complete
+// in issue order, so only the compiler must keep the accesses on their side
+// of this point.  Otherwise every thread of the workgroup waits here.
+template <bool WAVE>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham stage of radix R on an N = T*E point team transform, for NP
 // interleaved polarisations.  Butterfly b = j + u*T (u < E/R) reads elements
 // b + r*N/R, which are the thread's own slots u + r*E/R; it writes element r
 // to (b / NS)*NS*R + (b % NS) + r*NS.  The last stage (NS*R == N) writes back
 // into the same register slots, so no exchange follows it.  tw holds
 // W_TWN^m for m < TWN (TWN a multiple of N).
-template <int N, int E, int NP, int R, int NS, bool LAST, int TWN>
+template <int N, int E, int NP, int R, int NS, bool LAST, int TWN, bool WAVE>
 __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
                                       const float2 *__restrict__ tw) {
   constexpr int T = N / E;
@@ -156,10 +172,10 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
 #pragma unroll
         for (int r = 0; r < R; ++r) lds[lds_pos(base + r * NS)] = a[r];
       }
-      __syncthreads();
+      team_sync<WAVE>();
 #pragma unroll
       for (int m = 0; m < E; ++m) v[p][m] = lds[lds_pos(j + m * T)];
-      __syncthreads();
+      team_sync<WAVE>();
     }
     return;
   }
@@ -193,46 +209,49 @@ __device__ __forceinline__ void stage(float2 (&v)[NP][E], float2 *lds, int j,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    team_sync<WAVE>();
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
       for (int m = 0; m < E; ++m) v[p][m] = lds[p * LL + lds_pos(j + m * T)];
-    __syncthreads();
+    team_sync<WAVE>();
   }
 }
 
-template <int N, int E, int NP, int NS, int TWN>
+template <int N, int E, int NP, int NS, int TWN, bool WAVE>
 struct Stages {
   __device__ __forceinline__ static void run(float2 (&v)[NP][E], float2 *lds, int j,
                                              const float2 *__restrict__ tw) {
     if constexpr (NS * E == N) {
-      stage<N, E, NP, E, NS, true, TWN>(v, lds, j, tw);
+      stage<N, E, NP, E, NS, true, TWN, WAVE>(v, lds, j, tw);
     } else {
-      stage<N, E, NP, E, NS, false, TWN>(v, lds, j, tw);
-      Stages<N, E, NP, NS * E, TWN>::run(v, lds, j, tw);
+      stage<N, E, NP, E, NS, false, TWN, WAVE>(v, lds, j, tw);
+      Stages<N, E, NP, NS * E, TWN, WAVE>::run(v, lds, j, tw);
     }
   }
 };
 
 // Forward FFT of N points (power of two, E | N, N <= 4096) for NP
 // polarisations held by a team of N/E threads.  lds: NP*lds_len<N>() float2
-// private to the team.  tw: W_TWN^m table (global or LDS).  Every thread of
-// the workgroup must call it (barriers inside).
-template <int N, int E, int NP, int TWN = kTwN>
+// private to the team.  tw: W_TWN^m table (global or LDS).  WAVE: the team is
+// N/E consecutive lanes of one wavefront (N/E divides 64), so its exchanges
+// need no workgroup barrier; otherwise every thread of the workgroup must call
+// it (barriers inside).
+template <int N, int E, int NP, int TWN = kTwN, bool WAVE = false>
 __device__ __forceinline__ void team_fft(float2 (&v)[NP][E], float2 *lds, int j,
                                          const float2 *__restrict__ tw) {
+  static_assert(!WAVE || (64 % (N / E) == 0), "a wave-synchronised team must fit in a wavefront");
   if constexpr (N > 1) {
     constexpr int R0 = 1 << (ilog2(N) % ilog2(E));
     if constexpr (R0 > 1) {
       if constexpr (R0 == N) {
-        stage<N, E, NP, R0, 1, true, TWN>(v, lds, j, tw);
+        stage<N, E, NP, R0, 1, true, TWN, WAVE>(v, lds, j, tw);
       } else {
-        stage<N, E, NP, R0, 1, false, TWN>(v, lds, j, tw);
-        Stages<N, E, NP, R0, TWN>::run(v, lds, j, tw);
+        stage<N, E, NP, R0, 1, false, TWN, WAVE>(v, lds, j, tw);
+        Stages<N, E, NP, R0, TWN, WAVE>::run(v, lds, j, tw);
       }
     } else {
-      Stages<N, E, NP, 1, TWN>::run(v, lds, j, tw);
+      Stages<N, E, NP, 1, TWN, WAVE>::run(v, lds, j, tw);
     }
   }
 }

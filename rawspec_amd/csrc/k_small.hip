@@ -90,7 +90,8 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
 #pragma unroll
       for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
-    team_fft<N, E, NP, TwTab<N>::TWN>(v, lds + slot * team_stride<N, NP>(), j, twp);
+    // a team is T consecutive threads: within one wavefront when T <= 64
+    team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp);
     if (valid) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
