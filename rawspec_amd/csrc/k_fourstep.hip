@@ -11,9 +11,10 @@
 // (rawspec_gpu.cu:133-267, 1182-1195, 1555-1569) plus its accumulate kernel
 // and shift copies (:297-318, :1647-1697).
 //
-// Memory plan: the host launches A and B per chunk of channels sized so the
-// chunk's Y fits in the Infinity Cache, so B reads what A just wrote from
-// cache.  Y is tiled [k2 / TB][n1][k2 % TB][pol] so that a pass-B wave reads
+// Memory plan: the host launches A and B per chunk of channels whose Y fits
+// the work buffer (the whole product for the usual shapes: launches over
+// fewer channels leave the device underfilled, which costs more than the
+// Infinity Cache saves).  Y is tiled [k2 / TB][n1][k2 % TB][pol] so that a pass-B wave reads
 // 1 KB contiguous per load and a pass-A wave writes 256-byte runs.  For
 // integrations spanning several buffers pass B keeps the running sum in an
 // internal accumulator laid out one contiguous block per workgroup (fully
