@@ -47,6 +47,21 @@ constexpr int wg_team_np(int n) {
 constexpr int wg_team(int n) { return wg_team_np<2>(n); }
 constexpr int tile_of(int n) { return wg_team(n) / (n / elems_fs(n)); }
 
+// Offset within one spectrum's Y of the value (n1, k2 = j + m * T) in the
+// layout [k2 / TBY][n1][k2 % TBY]; linear in m when T is a multiple of TBY.
+template <int T, int TBY, int N1C>
+__device__ __forceinline__ unsigned y_off(unsigned j, unsigned n1, int m) {
+  if constexpr (T % TBY == 0) {
+    return ((j / TBY) * N1C + n1) * TBY + (j % TBY) + (unsigned)m * (T * N1C);
+  } else {
+    const unsigned k2 = j + m * T;
+    return ((k2 / TBY) * N1C + n1) * TBY + (k2 % TBY);
+  }
+}
+// Y layouts: tiled for the vector pass B (k2 tile tile_of(N1)), plain
+// [k2][n1] rows for the matrix-core pass B.
+enum { Y_TILED = 0, Y_ROWS = 1 };
+
 // Pass A for N = N1 * N2 with N1 = R1 * N2 (R1 = 1 or 2).  The four-step
 // twiddle W_N^e, e = n1 * k2 < N, is split as e = h * N1 + r:
 // W_N^(h * N1) = W_N2^h is the sub-transform's own table (already in LDS)
@@ -54,8 +69,8 @@ constexpr int tile_of(int n) { return wg_team(n) / (n / elems_fs(n)); }
 // N1 entries of the FP64-generated fine table; so each twiddle costs two LDS
 // reads and one complex product (two correctly rounded factors, as with the
 // global two-level table) instead of two scattered global gathers.
-template <int N2, int R1, int NBPS, int NP>
-__global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
+template <int N2, int R1, int NBPS, int NP, int YF>
+__global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   constexpr int E = elems_fs(N2);
   constexpr int T = N2 / E;
   constexpr int TA = tile_of(N2);      // n1 values per workgroup
@@ -103,14 +118,17 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
   }
   team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
 
-  // Y[cl][b * Nss + sp][k2 / TBy][n1][k2 % TBy][pol]
-  float2 *dst = a.work + ((unsigned long long)cl * nsp + b * a.Nss + sp) * N * NP;
+  // Y[cl][b * Nss + sp][k2 / TBY][n1][k2 % TBY][pol]; offsets within one
+  // spectrum (N <= 2^24 values) fit 32 bits
+  constexpr unsigned TBY = YF == Y_ROWS ? 1 : tile_of(N1C);
+  const unsigned long long spec = (unsigned long long)cl * nsp + b * a.Nss + sp;
+  float2 *dst = a.work + spec * N * NP;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const unsigned k2 = j + m * T;
     const unsigned e = n1 * k2;            // < N <= 2^24
     const float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
-    const unsigned long long off = ((unsigned long long)(k2 / TBy) * N1 + n1) * TBy + (k2 % TBy);
+    const unsigned off = y_off<T, TBY, N1C>(j, n1, m);
     if constexpr (NP == 2) {
       const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
       reinterpret_cast<float4 *>(dst)[off] = make_float4(y0.x, y0.y, y1.x, y1.y);
@@ -120,8 +138,10 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a, unsigned TBy) {
   }
 }
 
+// at most 256 registers per thread, so two waves share each SIMD
 template <int N1, int NP, int PMODE>
-__global__ __launch_bounds__(wg_team(N1)) void k_fs_b(FsArgs a) {
+__global__ __launch_bounds__(wg_team(N1)) __attribute__((amdgpu_waves_per_eu(2)))
+void k_fs_b(FsArgs a) {
   constexpr int E = elems_fs(N1);
   constexpr int T = N1 / E;
   constexpr int TB = tile_of(N1);      // k2 values per workgroup
@@ -216,46 +236,53 @@ __global__ __launch_bounds__(wg_team(N1)) void k_fs_b(FsArgs a) {
 int fs_tile_a(int n2) { return (int)(wg_team(n2) / (n2 / elems_fs(n2))); }
 int fs_tile_b(int n1) { return (int)(wg_team(n1) / (n1 / elems_fs(n1))); }
 
-template <int N2, int R1, int NBPS>
-static int go_a_r(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
+template <int N2, int R1, int NBPS, int YF>
+static void go_a_y(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
   constexpr int WGT = wg_team(N2);
-  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a, TBy);
-  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a, TBy);
+  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2, YF>), dim3(nb), dim3(WGT), 0, s, a);
+  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1, YF>), dim3(nb), dim3(WGT), 0, s, a);
+}
+
+template <int N2, int R1, int NBPS>
+static int go_a_r(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
+  if (a.tby == 1) go_a_y<N2, R1, NBPS, Y_ROWS>(np, a, nb, s);
+  else go_a_y<N2, R1, NBPS, Y_TILED>(np, a, nb, s);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 template <int N2, int NBPS>
-static int go_a(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
-  if (a.N1 == (unsigned)N2) return go_a_r<N2, 1, NBPS>(np, a, nb, TBy, s);
+static int go_a(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
+  if (a.N1 == (unsigned)N2) return go_a_r<N2, 1, NBPS>(np, a, nb, s);
   if constexpr (2 * N2 <= kTwN) {
-    if (a.N1 == 2u * N2) return go_a_r<N2, 2, NBPS>(np, a, nb, TBy, s);
+    if (a.N1 == 2u * N2) return go_a_r<N2, 2, NBPS>(np, a, nb, s);
   }
   return 2;
 }
 
 template <int NBPS>
-static int go_a_n(int np, const FsArgs &a, unsigned nb, unsigned TBy, hipStream_t s) {
+static int go_a_n(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
   switch (a.N2) {
-    case 64: return go_a<64, NBPS>(np, a, nb, TBy, s);
-    case 128: return go_a<128, NBPS>(np, a, nb, TBy, s);
-    case 256: return go_a<256, NBPS>(np, a, nb, TBy, s);
-    case 512: return go_a<512, NBPS>(np, a, nb, TBy, s);
-    case 1024: return go_a<1024, NBPS>(np, a, nb, TBy, s);
-    case 2048: return go_a<2048, NBPS>(np, a, nb, TBy, s);
-    case 4096: return go_a<4096, NBPS>(np, a, nb, TBy, s);
+    case 64: return go_a<64, NBPS>(np, a, nb, s);
+    case 128: return go_a<128, NBPS>(np, a, nb, s);
+    case 256: return go_a<256, NBPS>(np, a, nb, s);
+    case 512: return go_a<512, NBPS>(np, a, nb, s);
+    case 1024: return go_a<1024, NBPS>(np, a, nb, s);
+    case 2048: return go_a<2048, NBPS>(np, a, nb, s);
+    case 4096: return go_a<4096, NBPS>(np, a, nb, s);
   }
   return 2;
 }
 
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
-  const unsigned TBy = a.tby;
   const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
-  if (nb == 0 || nb > 0x7fffffffull || TBy == 0 || a.N2 % TBy != 0 || a.nbat == 0 ||
+  // Y's k2 tile: the vector pass B's, or 1 for the matrix-core pass B
+  const bool tiled = a.tby == (unsigned)fs_tile_b((int)a.N1);
+  if (nb == 0 || nb > 0x7fffffffull || !(tiled || a.tby == 1) || a.nbat == 0 ||
       a.nbat > (unsigned)kMaxBatch)
     return 2;
-  if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, TBy, s);
-  if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, TBy, s);
+  if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);
+  if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, s);
   return 2;
 }
 
