@@ -75,6 +75,24 @@ void *rawspec_amd_stream(rawspec_context *ctx);
 int rawspec_amd_set_mfma(rawspec_context *ctx, int enable);
 int rawspec_amd_get_mfma(rawspec_context *ctx);
 
+/* Held first-pass output.  A product longer than 4096 points whose
+ * integration spans several buffers transforms in two passes; its second
+ * pass adds into the running sum of the integration, a read-modify-write of
+ * the whole sum.  Called one buffer at a time (rawspec_start_processing(),
+ * rawspec_amd_start_processing_device()), such a product keeps each buffer's
+ * first-pass output in device memory and runs the second pass once per
+ * max_buffers buffers (and at the end of every integration), as a batch call
+ * would.  Results, dumps and callbacks are those of the one-buffer path; an
+ * input buffer may be reused as soon as its call's work completes.  Default
+ * 8; capped by the integration length and by 8 GiB per context, allocated
+ * at the first integration that holds and kept until cleanup (1 GiB per
+ * held buffer for 64 channels x 2^20 points x 2 pols).  max_buffers < 2
+ * turns it off.  A new value applies from the next integration start.
+ * Returns 0 on success; the getter returns the setting, -1 before
+ * initialisation. */
+int rawspec_amd_set_held_buffers(rawspec_context *ctx, unsigned int max_buffers);
+int rawspec_amd_get_held_buffers(rawspec_context *ctx);
+
 /* Two-lane execution: the four-step products' kernels run on one stream and
  * the small products' on another, both forked from and joined back into the
  * context's stream within each start call, so vector-bound small transforms

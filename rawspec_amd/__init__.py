@@ -208,6 +208,8 @@ def lib():
             ("rawspec_amd_profile", [P, C.c_int], C.c_int),
             ("rawspec_amd_set_mfma", [P, C.c_int], C.c_int),
             ("rawspec_amd_get_mfma", [P], C.c_int),
+            ("rawspec_amd_set_held_buffers", [P, C.c_uint], C.c_int),
+            ("rawspec_amd_get_held_buffers", [P], C.c_int),
             ("rawspec_amd_set_lanes", [P, C.c_int], C.c_int),
             ("rawspec_amd_get_lanes", [P], C.c_int),
             ("rawspec_amd_profile_report", [P, C.c_char_p, C.c_size_t], C.c_int),
@@ -326,6 +328,12 @@ class Spectrometer:
 
     def get_mfma(self) -> int:
         return lib().rawspec_amd_get_mfma(C.byref(self.ctx))
+
+    def set_held_buffers(self, max_buffers: int) -> int:
+        return lib().rawspec_amd_set_held_buffers(C.byref(self.ctx), max_buffers)
+
+    def get_held_buffers(self) -> int:
+        return lib().rawspec_amd_get_held_buffers(C.byref(self.ctx))
 
     def set_lanes(self, enable: bool) -> int:
         return lib().rawspec_amd_set_lanes(C.byref(self.ctx), 1 if enable else 0)

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   const unsigned cb = cs / a.Nss;
   const unsigned b = cb % a.nbat;      // buffer within the batch
   const unsigned cl = cb / a.nbat;     // channel within the chunk
-  const unsigned nsp = a.Nss * a.nbat; // spectra per channel in Y
+  const unsigned nsp = a.Nss * a.ynb;  // spectra per channel in Y
   const int tid = threadIdx.x;
   const int sg = tid % TA;             // n1 slot: fastest across lanes
   const int j = tid / TA;
@@ -118,10 +118,10 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   }
   team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
 
-  // Y[cl][b * Nss + sp][k2 / TBY][n1][k2 % TBY][pol]; offsets within one
-  // spectrum (N <= 2^24 values) fit 32 bits
+  // Y[cl][(yb0 + b) * Nss + sp][k2 / TBY][n1][k2 % TBY][pol]; offsets within
+  // one spectrum (N <= 2^24 values) fit 32 bits
   constexpr unsigned TBY = YF == Y_ROWS ? 1 : tile_of(N1C);
-  const unsigned long long spec = (unsigned long long)cl * nsp + b * a.Nss + sp;
+  const unsigned long long spec = (unsigned long long)cl * nsp + (a.yb0 + b) * a.Nss + sp;
   float2 *dst = a.work + spec * N * NP;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -172,7 +172,7 @@ void k_fs_b(FsArgs a) {
 
   for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
     // this workgroup's tile: N1 rows of TB k2 values, contiguous
-    const float2 *y = a.work + (((unsigned long long)cl * a.Nss * a.nbat + sp) * N +
+    const float2 *y = a.work + (((unsigned long long)cl * a.Nss * a.ynb + sp) * N +
                                 (unsigned long long)tile * N1 * TB) * NP;
     float2 v[NP][E];
 #pragma unroll
@@ -279,7 +279,7 @@ int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   // Y's k2 tile: the vector pass B's, or 1 for the matrix-core pass B
   const bool tiled = a.tby == (unsigned)fs_tile_b((int)a.N1);
   if (nb == 0 || nb > 0x7fffffffull || !(tiled || a.tby == 1) || a.nbat == 0 ||
-      a.nbat > (unsigned)kMaxBatch)
+      a.nbat > (unsigned)kMaxBatch || a.yb0 + a.nbat > a.ynb)
     return 2;
   if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);
   if (nbps == 16) return go_a_n<16>(np, a, (unsigned)nb, s);
@@ -306,7 +306,7 @@ static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) 
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N2 / fs_tile_b((int)a.N1);
   const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.nch;
-  if (nb == 0 || nb > 0x7fffffffull) return 2;
+  if (nb == 0 || nb > 0x7fffffffull || a.nbat == 0 || a.nbat > a.ynb) return 2;
   switch (a.N1) {
     case 128: return go_b<128>(np, pmode, a, (unsigned)nb, s);
     case 256: return go_b<256>(np, pmode, a, (unsigned)nb, s);

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void
   float acc[16][Q];
   const float s1 = a.mscale;
   const float4 *y0 = reinterpret_cast<const float4 *>(a.work) +
-                     ((unsigned long long)cl * a.Nss * a.nbat + g * a.na_buf) * N +
+                     ((unsigned long long)cl * a.Nss * a.ynb + g * a.na_buf) * N +
                      (unsigned long long)k2 * N1;
   if constexpr (ONE) {
     fs_b_column<PMODE, true>(y0, s1, a.gam, K, acc);
@@ -298,7 +298,9 @@ static int go_b_mfma(const FsArgs &a, unsigned nb, hipStream_t s) {
 int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
 
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {
-  if (!fs_b_mfma_ok((int)a.N1, np) || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f)) return 2;
+  if (!fs_b_mfma_ok((int)a.N1, np) || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f) ||
+      a.nbat == 0 || a.nbat > a.ynb)
+    return 2;
   const unsigned long long nb = (unsigned long long)(a.N2 / kMfmaWaves) * a.Ng * a.nch;
   if (nb == 0 || nb > 0x7fffffffull) return 2;
   switch (pmode) {

@@ -69,6 +69,10 @@ struct FsArgs {
   float *acc_int;                 // internal accumulator (Nis > 1)
   const float2 *tw;               // W_4096^m
   const float2 *twf;              // W_N^r, r < N1 (N1 <= 4096)
+  // Y holds ynb buffers' spectra per channel ([cl][slot * Nss + sp]); pass A
+  // writes this launch's nbat buffers to slots yb0 .., pass B reads slots
+  // 0 .. nbat - 1
+  unsigned yb0, ynb;
   unsigned tby;                   // k2 tile of Y: fs_tile_b(N1) for k_fs_b, 1 for k_fs_b_mfma
   float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
 };

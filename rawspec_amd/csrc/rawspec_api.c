@@ -645,6 +645,20 @@ int rawspec_amd_get_mfma(rawspec_context *ctx)
   return rsd_get_mfma(g->dev);
 }
 
+int rawspec_amd_set_held_buffers(rawspec_context *ctx, unsigned int max_buffers)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  return rsd_set_hold(g->dev, max_buffers);
+}
+
+int rawspec_amd_get_held_buffers(rawspec_context *ctx)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return -1;
+  return (int)rsd_get_hold(g->dev);
+}
+
 int rawspec_amd_profile_report(rawspec_context *ctx, char *buf, size_t len)
 {
   rs_gpu_ctx *g = rs_get(ctx);

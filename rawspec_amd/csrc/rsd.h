@@ -117,6 +117,12 @@ int rsd_set_mfma(rsd_ctx *d, int enable);
 int rsd_set_lanes(rsd_ctx *d, int enable);
 int rsd_get_lanes(rsd_ctx *d);
 int rsd_get_mfma(rsd_ctx *d);
+/* Buffers of first-pass output a four-step product that integrates over
+ * several buffers may hold across one-buffer calls (default 8, capped by the
+ * product's integration length and 8 GiB per context; below 2: none).
+ * Applies from the next integration start. */
+int rsd_set_hold(rsd_ctx *d, unsigned max_buffers);
+unsigned rsd_get_hold(rsd_ctx *d);
 
 #ifdef __cplusplus
 }

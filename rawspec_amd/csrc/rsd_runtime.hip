@@ -11,6 +11,10 @@
 //   d_part  partial integrations of groups split over workgroups
 //   d_work  four-step intermediate of one channel chunk (several buffers'
 //           spectra when batched), tiled for pass B
+//   d_held  per four-step product integrating over several buffers, the
+//           intermediate of up to hold_k buffers over all channels, so that
+//           one-buffer calls run pass B once per hold_k buffers (allocated
+//           at the first integration that uses it)
 //   d_tw    W_4096^m table; per four-step product the fine table W_N^r,
 //           r < N1.  All twiddles are computed in FP64 on the host and
 //           rounded once to FP32.
@@ -56,6 +60,15 @@ struct ProdPlan {
   unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
   unsigned fs_bmax;        // buffers per batched launch pair (1: no batching)
   int fs_mfma;             // pass B on the matrix cores for the current integration
+  // held first-pass output (one-buffer calls of an integration over several
+  // buffers): capacity hold_k buffers; `holding` is latched per integration
+  unsigned hold_k;
+  unsigned hold_use;       // slots used in the current integration (<= hold_k)
+  float2 *d_held;
+  int hold_failed;         // allocation refused: not retried
+  int holding;
+  unsigned held;           // buffers whose pass B is pending
+  int held_first;          // the pending buffers start the integration
   float2 *d_twf;           // W_N^r, r < N1 (pass A's fine twiddle table)
   float *d_acc_int;        // running sum across buffers (Nis > 1)
   // outputs
@@ -77,6 +90,12 @@ struct ProdPlan {
 constexpr size_t kFsWorkBytes = (size_t)RSD_FS_WORK_MB << 20;
 // smallest input buffer for which a context runs its products on two lanes
 constexpr size_t kLaneMinBytes = (size_t)64 << 20;
+// Held first-pass output per context, at most (one-buffer calls): 8 buffers
+// of a 256 MiB, 2^20-point product
+#ifndef RSD_HOLD_MB
+#define RSD_HOLD_MB 8192
+#endif
+constexpr size_t kHoldBytes = (size_t)RSD_HOLD_MB << 20;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -116,6 +135,8 @@ struct rsd_ctx {
   size_t part_len;
   float2 *d_work;
   size_t work_len;
+  unsigned hold_max;                   // buffers a product may hold (rsd_set_hold)
+  size_t held_bytes;                   // allocated for held output
   float2 *d_tw;
   float *d_w;
   ProdPlan pp[RSD_MAX_PRODUCTS];
@@ -262,6 +283,7 @@ void rsd_destroy(rsd_ctx *d) {
     if (pp.d_ics) (void)hipFree(pp.d_ics);
     if (pp.d_twf) (void)hipFree(pp.d_twf);
     if (pp.d_acc_int) (void)hipFree(pp.d_acc_int);
+    if (pp.d_held) (void)hipFree(pp.d_held);
   }
   if (d->d_in) (void)hipFree(d->d_in);
   if (d->d_stage) (void)hipFree(d->d_stage);
@@ -333,6 +355,14 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
       pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
       pp.Ng = pp.p.Nss / pp.na_buf;
       pp.fs_chunk = fs_chunk_for(d, pp, 1);
+      pp.hold_k = 0;
+      if (pp.p.Nis > 1) {
+        const size_t per_buf = (size_t)cfg->Nc * pp.p.Nss * N * cfg->Np * sizeof(float2);
+        size_t k = kHoldBytes / per_buf;
+        k = k > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : k;
+        k = k > pp.p.Nis ? pp.p.Nis : k;
+        pp.hold_k = k >= 2 ? (unsigned)k : 0;
+      }
       // batches of consecutive buffers only where an integration spans
       // several buffers (then each dump group is the whole buffer)
       pp.fs_bmax = 1;
@@ -430,6 +460,8 @@ static rsd_ctx *new_ctx() {
   d->d_in = d->d_stage = nullptr;
   d->d_part = nullptr;
   d->d_work = d->d_tw = nullptr;
+  d->hold_max = rsk::kMaxBatch;
+  d->held_bytes = 0;
   d->d_w = nullptr;
   d->prof = false;
   d->mfma = 0;
@@ -485,6 +517,9 @@ int rsd_zero_block(rsd_ctx *d, unsigned dblk) {
 
 static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n, float isign,
                         float gam, float scale, int first, int last);
+static int hold_ready(rsd_ctx *d, ProdPlan &pp);
+static int run_fourstep_held(rsd_ctx *d, int i, const uint8_t *in, float isign, float gam,
+                             float scale, int first, int last);
 
 int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last) {
   RSD_CHECK(hipSetDevice(d->dev));
@@ -556,6 +591,8 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
     }
     return 0;
   }
+  if (first) pp.holding = hold_ready(d, pp);
+  if (pp.holding) return run_fourstep_held(d, i, in, isign, gam, scale, first, last);
   return run_fourstep(d, i, &in, 1, isign, gam, scale, first, last);
 }
 
@@ -563,13 +600,12 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
 // pairs of up to fs_bmax buffers (batched: pass B sums their spectra before
 // touching the running sum), each over channel chunks sized so the chunk's Y
 // stays within its budget.
-static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n, float isign,
-                        float gam, float scale, int first, int last) {
-  ProdPlan &pp = d->pp[i];
+// Arguments shared by every pass A/B launch of product i; latches the
+// integration's pass-B engine when it starts.
+static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float gam, float scale,
+                    int first) {
   const rsd_config &c = d->cfg;
   const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
-  const hipStream_t s = prod_stream(d, pp);
-  rsk::FsArgs a;
   a.chan_stride = d->chan_stride;
   a.Nc = c.Nc;
   a.Nss = pp.p.Nss;
@@ -594,6 +630,103 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
     while (ldexp((double)pp.N2 * full * 1.4142135623730951, -e) > 32768.0) ++e;
     a.mscale = (float)ldexp(1.0, -e);
   }
+  a.yb0 = 0;
+}
+
+static int launch_pass_b(rsd_ctx *d, int i, const rsk::FsArgs &a, hipStream_t s) {
+  ProdPlan &pp = d->pp[i];
+  if (pp.fs_mfma) {
+    ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt, s);
+    return rsk::launch_fs_b_mfma((int)d->cfg.Np, pp.p.npolout, a, s);
+  }
+  ProfScope ps(d, i, "k_fs_b", pp.p.Nt, s);
+  return rsk::launch_fs_b((int)d->cfg.Np, pp.p.npolout, a, s);
+}
+
+// Pass B over the held buffers of product i, which complete the integration
+// when `last`.
+static int flush_held(rsd_ctx *d, int i, rsk::FsArgs &a, int last) {
+  ProdPlan &pp = d->pp[i];
+  if (pp.held == 0) return 0;
+  a.work = pp.d_held;
+  a.ynb = pp.hold_k;
+  a.c0 = 0;
+  a.nch = d->cfg.Nc;
+  a.nbat = pp.held;
+  a.na_buf = pp.held * pp.na_buf;
+  a.Ng = pp.Ng;
+  if (pp.held_first) a.mode = last ? rsk::FS_HOST : rsk::FS_FIRST;
+  else a.mode = last ? rsk::FS_LAST : rsk::FS_MID;
+  pp.held = 0;
+  pp.held_first = 0;
+  return launch_pass_b(d, i, a, prod_stream(d, pp));
+}
+
+// Whether product i holds first-pass output through the integration that
+// starts now; allocates the product's slots on first use (not retried after
+// a refusal).  Latches the number of slots used.
+static int hold_ready(rsd_ctx *d, ProdPlan &pp) {
+  const unsigned k = pp.hold_k < d->hold_max ? pp.hold_k : d->hold_max;
+  pp.held = 0;
+  if (k < 2 || pp.hold_failed) return 0;
+  if (!pp.d_held) {
+    const size_t bytes = (size_t)pp.hold_k * d->cfg.Nc * pp.p.Nss * pp.p.Nt * d->cfg.Np *
+                         sizeof(float2);
+    if (d->held_bytes + bytes > kHoldBytes || hipMalloc(&pp.d_held, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      pp.d_held = nullptr;
+      pp.hold_failed = 1;
+      return 0;
+    }
+    d->held_bytes += bytes;
+  }
+  pp.hold_use = k;
+  return 1;
+}
+
+// One buffer of product i while it holds first-pass output: pass A into the
+// next slot, pass B over the slots when hold_use are full or the integration
+// completes.
+static int run_fourstep_held(rsd_ctx *d, int i, const uint8_t *in, float isign, float gam,
+                             float scale, int first, int last) {
+  ProdPlan &pp = d->pp[i];
+  const hipStream_t s = prod_stream(d, pp);
+  rsk::FsArgs a;
+  fs_args(d, pp, a, isign, gam, scale, first);
+  if (first) {
+    pp.held = 0;
+    pp.held_first = 1;
+  }
+  a.work = pp.d_held;
+  a.ynb = pp.hold_k;
+  a.yb0 = pp.held;
+  a.c0 = 0;
+  a.nch = d->cfg.Nc;
+  a.nbat = 1;
+  for (unsigned b = 0; b < (unsigned)rsk::kMaxBatch; ++b) a.in[b] = in;
+  a.na_buf = pp.na_buf;
+  a.Ng = pp.Ng;
+  a.mode = rsk::FS_MID;      // pass A does not read it
+  {
+    ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
+    if (rsk::launch_fs_a((int)d->cfg.Nbps, (int)d->cfg.Np, a, s)) return 1;
+  }
+  ++pp.held;
+  if (!last && pp.held < pp.hold_use) return 0;
+  return flush_held(d, i, a, last);
+}
+
+static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n, float isign,
+                        float gam, float scale, int first, int last) {
+  ProdPlan &pp = d->pp[i];
+  const rsd_config &c = d->cfg;
+  const hipStream_t s = prod_stream(d, pp);
+  rsk::FsArgs a;
+  fs_args(d, pp, a, isign, gam, scale, first);
+  // buffers held by earlier one-buffer calls of this integration go first
+  if (first) pp.held = 0;
+  if (pp.held && flush_held(d, i, a, 0)) return 1;
+  a.work = d->d_work;
   for (unsigned j = 0; j < n;) {
     const unsigned kb = n - j < pp.fs_bmax ? n - j : pp.fs_bmax;
     const bool f = first && j == 0, l = last && j + kb == n;
@@ -602,6 +735,7 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
     else if (l) a.mode = rsk::FS_LAST;
     else a.mode = rsk::FS_MID;
     a.nbat = kb;
+    a.ynb = kb;
     for (unsigned b = 0; b < (unsigned)rsk::kMaxBatch; ++b) a.in[b] = ins[j + (b < kb ? b : 0)];
     a.na_buf = kb * pp.na_buf;
     a.Ng = pp.Ng;
@@ -614,13 +748,7 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
         ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
         if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, s)) return 1;
       }
-      if (pp.fs_mfma) {
-        ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt, s);
-        if (rsk::launch_fs_b_mfma((int)c.Np, pp.p.npolout, a, s)) return 1;
-      } else {
-        ProfScope ps(d, i, "k_fs_b", pp.p.Nt, s);
-        if (rsk::launch_fs_b((int)c.Np, pp.p.npolout, a, s)) return 1;
-      }
+      if (launch_pass_b(d, i, a, s)) return 1;
     }
     j += kb;
   }
@@ -641,6 +769,9 @@ int rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n
   const float isign = inverse ? -1.0f : 1.0f;
   const float gam = (c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f);
   const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
+  // an integration started by a batch is batched throughout; one-buffer
+  // calls continuing it run pass A/B per buffer
+  if (first) pp.holding = 0;
   return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, isign, gam,
                       1.0f / (full * full), first, last);
 }
@@ -740,6 +871,13 @@ int rsd_set_mfma(rsd_ctx *d, int enable) {
 }
 
 int rsd_get_mfma(rsd_ctx *d) { return d->mfma; }
+
+int rsd_set_hold(rsd_ctx *d, unsigned max_buffers) {
+  d->hold_max = max_buffers;
+  return 0;
+}
+
+unsigned rsd_get_hold(rsd_ctx *d) { return d->hold_max; }
 
 int rsd_set_lanes(rsd_ctx *d, int enable) {
   if (enable && !(d->lane[0] && d->lane[1])) return 1;
