@@ -22,7 +22,9 @@ extern "C" {
 /* Validates and derives exactly as rawspec_initialize() does (updating the
  * same fields of ctx) but allocates nothing; reports the device bytes and the
  * pinned host bytes rawspec_initialize() would allocate.  For admission
- * control by HBM budget.  Returns 0 when the configuration is valid. */
+ * control by HBM budget.  Held first-pass output
+ * (rawspec_amd_set_held_buffers) is allocated later, at first use, and not
+ * included.  Returns 0 when the configuration is valid. */
 int rawspec_amd_plan(rawspec_context *ctx, size_t *device_bytes, size_t *host_bytes);
 
 /* Device pointer and size of the context's input buffer, laid out
