@@ -6,7 +6,7 @@ counts, callback order and channel order must match exactly.
 import numpy as np
 import pytest
 
-from tests.spectro_cases import PLAIN_TOL, check_results, run_case
+from tests.spectro_cases import MFMA_FLOOR_MULT, PLAIN_TOL, check_results, run_case
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +79,27 @@ def test_sixteen_antennas_int16_four_products_ics():
     assert [cfg.Nis[i] for i in range(4)] == [1, 1, 2, 1]
     assert sorted((n, i) for (n, i, *_r) in res) == [(0, 0), (0, 1), (0, 3), (1, 0), (1, 1),
                                                       (1, 2), (1, 3)]
+
+
+@pytest.mark.parametrize("mfma", [False, True])
+def test_stokes_three_products_full_integration_lengths(mfma):
+    """The 64-channel Stokes product set at its full integration lengths on
+    one of its channels (channels are independent; the tone's): blocks of
+    524288 samples, two per buffer, Nfft {2^20, 8, 1024} x Nint {51, 128,
+    3072} over 51 buffers, so the 2^20 product integrates across 51 buffers
+    (first-pass output held across the one-buffer calls) and the 1024-point
+    product over 3072 spectra across 3 buffers; on the matrix cores, 9
+    buffers with Nint {9, 128, 3072}.  FP32 sums in blocks (registers, a
+    fixed-order tree across slots, one add per buffer or held group) keep
+    the 3072-spectrum integration within the plain bar without compensated
+    summation."""
+    nbuf = 9 if mfma else 51
+    res, cfg, ev = run_case(Nc=1, Ntpb=524288, Nts=[1 << 20, 8, 1024], Nas=[nbuf, 128, 3072],
+                            Npolout=[-4, -4, -4], nbuf=nbuf, tone=(0, 20.0, 300.0, 1024.0),
+                            mfma=mfma)
+    worst = check_results(res, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT if mfma else 2.0)
+    print(f"  dumps={len(res)} worst={ {k: f'{v:.2e}' for k, v in worst.items()} }", flush=True)
+    assert [sum(1 for r in res if r[1] == i) for i in range(3)] == [1, nbuf, nbuf // 3]
 
 
 def test_single_pol():
