@@ -210,6 +210,12 @@ def other_config(torch, R, W, nring=8):
     out = {"GBps": round(steps * buf_bytes / dt / 1e9, 2), "ms_per_step": round(ms, 4),
            "step_bytes": buf_bytes, "steps": steps,
            "pipeline_frac_of_hbm": round(B_alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    # the same buffers 8 per call (rawspec_amd_start_processing_device_batch)
+    sp.reset_integration()
+    sb = (steps + 7) // 8 * 8
+    dt8, _ = timed(torch, sp, ring, sb, warmup=8, batch=8)
+    out["served_GBps"] = round(sb * buf_bytes / dt8 / 1e9, 2)
+    out["served_ms_per_step"] = round(dt8 / sb * 1e3, 4)
     sp.close()
     del ring
     torch.cuda.empty_cache()

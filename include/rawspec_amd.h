@@ -95,6 +95,29 @@ int rawspec_amd_get_mfma(rawspec_context *ctx);
 int rawspec_amd_set_held_buffers(rawspec_context *ctx, unsigned int max_buffers);
 int rawspec_amd_get_held_buffers(rawspec_context *ctx);
 
+/* Launch-plan autotuner.  Times candidate launch plans of every product by
+ * running the context's own start calls on the device buffer d_in (NULL:
+ * the context's input buffer; the data does not affect timing), per_call
+ * buffers per call (1: as rawspec_start_processing() and
+ * rawspec_amd_start_processing_device(); more: as the batch call), lanes and
+ * held output as configured, and keeps for each product in turn the plan
+ * under which the whole sequence runs fastest (a candidate must win by 3%).
+ * Each timing is `rounds` rounds (0: 3) of 16 buffers after a warm-up round.
+ * Small products try the workgroups a launch aims for and the slots per dump
+ * group (on the matrix-core engine, the wavefronts a launch aims for);
+ * four-step products try the intermediate's budget per launch pair, which
+ * sets the channels per launch and the buffers per batch.  Plans change only
+ * how work is split over workgroups and launches, so results stay within
+ * the same error bars.  The timing runs make no dumps and fire no callbacks;
+ * they overwrite the integration buffers, so every integration restarts
+ * with the next buffer, as after rawspec_reset_integration().  May grow the
+ * context's scratch buffers.  Writes one line per product ("p<i> N<Nt>
+ * <plan>: <best> ms (default <ms> ms, <n> candidates)", milliseconds per
+ * buffer of the whole sequence) into report (NUL-terminated; NULL with len 0
+ * for none).  Returns 0 on success. */
+int rawspec_amd_autotune(rawspec_context *ctx, const void *d_in, unsigned int per_call,
+                         unsigned int rounds, char *report, size_t len);
+
 /* Two-lane execution: the four-step products' kernels run on one stream and
  * the small products' on another, both forked from and joined back into the
  * context's stream within each start call, so vector-bound small transforms

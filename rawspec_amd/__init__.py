@@ -209,6 +209,8 @@ def lib():
             ("rawspec_amd_set_mfma", [P, C.c_int], C.c_int),
             ("rawspec_amd_get_mfma", [P], C.c_int),
             ("rawspec_amd_set_held_buffers", [P, C.c_uint], C.c_int),
+            ("rawspec_amd_autotune", [P, C.c_void_p, C.c_uint, C.c_uint, C.c_char_p, C.c_size_t],
+             C.c_int),
             ("rawspec_amd_get_held_buffers", [P], C.c_int),
             ("rawspec_amd_set_lanes", [P, C.c_int], C.c_int),
             ("rawspec_amd_get_lanes", [P], C.c_int),
@@ -328,6 +330,13 @@ class Spectrometer:
 
     def get_mfma(self) -> int:
         return lib().rawspec_amd_get_mfma(C.byref(self.ctx))
+
+    def autotune(self, dev_ptr: int | None = None, per_call: int = 1, rounds: int = 3) -> str:
+        """Runs rawspec_amd_autotune; returns its report (raises on failure)."""
+        buf = C.create_string_buffer(4096)
+        if lib().rawspec_amd_autotune(C.byref(self.ctx), dev_ptr, per_call, rounds, buf, len(buf)):
+            raise RuntimeError("rawspec_amd_autotune failed")
+        return buf.value.decode()
 
     def set_held_buffers(self, max_buffers: int) -> int:
         return lib().rawspec_amd_set_held_buffers(C.byref(self.ctx), max_buffers)

@@ -11,6 +11,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "rawspec.h"
 #include "rawspec_amd.h"
@@ -34,6 +35,7 @@ typedef struct {
   int caller_managed;             /* h_blkbufs belong to the caller      */
   int expand4;                    /* Nbps was 4                          */
   int device_output;              /* dumps stay in device memory         */
+  int tuning;                     /* autotuning: work without dumps      */
   unsigned int Nss[MAX_OUTPUTS];
   unsigned int Nis[MAX_OUTPUTS];
   rs_cb_t pre[MAX_OUTPUTS];
@@ -457,7 +459,7 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
     }
   }
   for (i = 0; i < ctx->No; i++)
-    if (last[i] && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i))) return 1;
+    if (last[i] && !g->tuning && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i))) return 1;
   return rsd_lane_end(g->dev, -1);
 }
 
@@ -485,8 +487,9 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
         return 1;
       }
       /* the dump reads the integration buffer the lane's next piece rewrites */
-      if (last && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i) ||
-                   (k + len < nbuf && rsd_lane_resume(g->dev, (int)i))))
+      if (last && !g->tuning &&
+          (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i) ||
+           (k + len < nbuf && rsd_lane_resume(g->dev, (int)i))))
         return 1;
       k += len;
     }
@@ -643,6 +646,87 @@ int rawspec_amd_get_mfma(rawspec_context *ctx)
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return -1;
   return rsd_get_mfma(g->dev);
+}
+
+/* Seconds per buffer of `rounds` rounds of RS_TUNE_BUFS buffers through the
+ * start calls a caller feeding `per_call` buffers per call makes, after one
+ * round of warm-up; each round starts new integrations. */
+#define RS_TUNE_BUFS 16
+static double rs_tune_time(rawspec_context *ctx, rs_gpu_ctx *g, const void *d_in,
+                           unsigned int per_call, unsigned int rounds)
+{
+  const void *ins[RS_TUNE_BUFS];
+  struct timespec t0, t1;
+  unsigned int r, k;
+  for (k = 0; k < RS_TUNE_BUFS; k++) ins[k] = d_in;
+  for (r = 0; r <= rounds; r++) {
+    if (r == 1) {
+      if (rsd_sync(g->dev)) return -1.0;
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+    }
+    g->inbuf_count = 0;
+    for (k = 0; k < RS_TUNE_BUFS; k += per_call) {
+      const unsigned int n = RS_TUNE_BUFS - k < per_call ? RS_TUNE_BUFS - k : per_call;
+      if (n == 1 ? rs_start(ctx, d_in, RAWSPEC_FORWARD_FFT)
+                 : rs_start_batch(ctx, (const void *const *)(ins + k), n, RAWSPEC_FORWARD_FFT))
+        return -1.0;
+    }
+  }
+  if (rsd_sync(g->dev)) return -1.0;
+  clock_gettime(CLOCK_MONOTONIC, &t1);
+  return ((double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec)) /
+         ((double)rounds * RS_TUNE_BUFS);
+}
+
+/* Coordinate descent over the products' launch-plan candidates: each
+ * product in turn takes the candidate under which the whole call sequence
+ * runs fastest, the other products keeping their current plans. */
+int rawspec_amd_autotune(rawspec_context *ctx, const void *d_in, unsigned int per_call,
+                         unsigned int rounds, char *report, size_t len)
+{
+  unsigned int i;
+  size_t off = 0;
+  int rc = 0;
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  if (!d_in) d_in = rsd_input_ptr(g->dev);
+  if (per_call == 0) per_call = 1;
+  if (per_call > RS_TUNE_BUFS) per_call = RS_TUNE_BUFS;
+  if (rounds == 0) rounds = 3;
+  if (report && len) report[0] = '\0';
+  if (rsd_sync(g->dev)) return 1;
+  g->tuning = 1;
+  for (i = 0; i < ctx->No && !rc; i++) {
+    const int n = rsd_tune_count(g->dev, (int)i);
+    int k, best = 0;
+    double t_best, t_dflt;
+    char plan[160];
+    rc = rsd_tune_apply(g->dev, (int)i, 0);
+    t_dflt = t_best = rc ? -1.0 : rs_tune_time(ctx, g, d_in, per_call, rounds);
+    if (t_best < 0) rc = 1;
+    for (k = 1; k < n && !rc; k++) {
+      double t;
+      const int a = rsd_tune_apply(g->dev, (int)i, k);
+      if (a == 2) continue;              /* the same plan as an earlier candidate */
+      if (a) { rc = 1; break; }
+      t = rs_tune_time(ctx, g, d_in, per_call, rounds);
+      if (t < 0) { rc = 1; break; }
+      /* a new plan must win clearly: timing noise is a few percent */
+      if (t < 0.97 * t_best) { t_best = t; best = k; }
+    }
+    if (!rc) rc = rsd_tune_apply(g->dev, (int)i, best) == 1;
+    if (!rc && report && len > off) {
+      int w;
+      rsd_tune_describe(g->dev, (int)i, plan, sizeof plan);
+      w = snprintf(report + off, len - off, "p%u N%u %s: %.4f ms (default %.4f ms, %d candidates)\n",
+                   i, ctx->Nts[i], plan, t_best * 1e3, t_dflt * 1e3, n);
+      if (w > 0) off += (size_t)w < len - off ? (size_t)w : len - off - 1;
+    }
+  }
+  g->tuning = 0;
+  /* the timing runs overwrote the integration buffers */
+  g->inbuf_count = 0;
+  return rc;
 }
 
 int rawspec_amd_set_held_buffers(rawspec_context *ctx, unsigned int max_buffers)

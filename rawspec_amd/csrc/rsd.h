@@ -123,6 +123,13 @@ int rsd_get_mfma(rsd_ctx *d);
  * Applies from the next integration start. */
 int rsd_set_hold(rsd_ctx *d, unsigned max_buffers);
 unsigned rsd_get_hold(rsd_ctx *d);
+/* Launch-plan candidates of product i (index 0: the defaults).
+ * rsd_tune_apply makes candidate k the product's plan (growing scratch
+ * buffers as needed, with the device idle); returns 2 when its plan equals
+ * an earlier candidate's.  rsd_tune_describe writes the current plan. */
+int rsd_tune_count(rsd_ctx *d, int i);
+int rsd_tune_apply(rsd_ctx *d, int i, int k);
+int rsd_tune_describe(rsd_ctx *d, int i, char *buf, size_t len);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -60,6 +61,11 @@ struct ProdPlan {
   unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
   unsigned fs_bmax;        // buffers per batched launch pair (1: no batching)
   int fs_mfma;             // pass B on the matrix cores for the current integration
+  // launch-plan tunables (defaults below; rsd_autotune times alternatives)
+  unsigned t_target;       // small kernel: workgroups a launch should reach
+  unsigned t_maxspg;       // small kernel: most slots per dump group
+  unsigned t_mtarget;      // matrix-core small kernel: wavefronts a launch should reach
+  size_t fs_budget;        // four-step: bytes of Y per pass A/B launch pair
   // held first-pass output (one-buffer calls of an integration over several
   // buffers): capacity hold_k buffers; `holding` is latched per integration
   unsigned hold_k;
@@ -199,18 +205,20 @@ static void plan_small(rsd_ctx *d, ProdPlan &pp) {
   pp.Ng = pp.p.Nss / pp.na_buf;
   pp.chunk = pp.na_buf;
   // Split long groups over workgroups until the launch fills the device
-  // (about 2048 workgroups for 256 CUs), keeping >= one spectrum per slot.
-  const unsigned long long target = 2048;
+  // (by default about 2048 workgroups for 256 CUs), keeping >= one spectrum
+  // per slot.
+  const unsigned long long target = pp.t_target;
   while ((unsigned long long)Nc * pp.Ng * (pp.na_buf / pp.chunk) < target && pp.chunk % 2 == 0 &&
          pp.chunk / 2 >= slots)
     pp.chunk /= 2;
   pp.G = pp.na_buf / pp.chunk;
   pp.direct = pp.G == 1;
   // Slots per group: a power of two, no larger than the group and at most
-  // kMaxSpg, so each slot integrates several spectra in registers and the
-  // final cross-slot sum stays short; the remaining slots take further
-  // groups (gpw groups per workgroup, only when the group is not split).
-  const unsigned kMaxSpg = 16;
+  // t_maxspg (default 16), so each slot integrates several spectra in
+  // registers and the final cross-slot sum stays short; the remaining slots
+  // take further groups (gpw groups per workgroup, only when the group is
+  // not split).
+  const unsigned kMaxSpg = pp.t_maxspg;
   pp.gpw = 1;
   pp.spg = slots;
   if (pp.G == 1) {
@@ -224,12 +232,13 @@ static void plan_small(rsd_ctx *d, ProdPlan &pp) {
   pp.bpc = ((pp.Ng + pp.gpw - 1) / pp.gpw) * pp.G;
 }
 
-// Matrix-core small kernel plan: halve the chunks until about 2048
-// wavefronts (eight per CU) are busy, keeping >= 8 spectra per wavefront.
+// Matrix-core small kernel plan: halve the chunks until t_mtarget
+// wavefronts (default 2048, eight per CU) are busy, keeping >= 8 spectra per
+// wavefront.
 static void plan_small_mfma(rsd_ctx *d, ProdPlan &pp) {
   pp.m_ok = rsk::small_mfma_ok((int)pp.p.Nt, (int)d->cfg.Np);
   pp.m_chunk = pp.na_buf;
-  while ((unsigned long long)d->cfg.Nc * pp.Ng * (pp.na_buf / pp.m_chunk) < 2048 &&
+  while ((unsigned long long)d->cfg.Nc * pp.Ng * (pp.na_buf / pp.m_chunk) < pp.t_mtarget &&
          pp.m_chunk % 2 == 0 && pp.m_chunk / 2 >= 8)
     pp.m_chunk /= 2;
   pp.m_G = pp.na_buf / pp.m_chunk;
@@ -309,11 +318,53 @@ void rsd_destroy(rsd_ctx *d) {
 }
 
 // Channels per pass A/B launch pair for launches of kb buffers: as many as
-// keep the chunk's intermediate within kFsWorkBytes (at least one).
+// keep the chunk's intermediate within the product's fs_budget (at least one).
 static unsigned fs_chunk_for(const rsd_ctx *d, const ProdPlan &pp, unsigned kb) {
   const size_t per_chan = (size_t)kb * pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
-  const size_t ch = kFsWorkBytes / per_chan;
+  const size_t ch = pp.fs_budget / per_chan;
   return (unsigned)(ch < 1 ? 1 : (ch > d->cfg.Nc ? d->cfg.Nc : ch));
+}
+
+// Launch plan of one product from its tunables; returns the float2
+// elements of the four-step intermediate it needs (0 for small products).
+static size_t plan_product(rsd_ctx *d, ProdPlan &pp) {
+  const rsd_config *cfg = &d->cfg;
+  const unsigned N = pp.p.Nt;
+  size_t work = 0;
+  if (pp.small) {
+    plan_small(d, pp);
+    plan_small_mfma(d, pp);
+  } else {
+    const unsigned L = ilog2u(N);
+    pp.N1 = 1u << ((L + 1) / 2);
+    pp.N2 = 1u << (L / 2);
+    pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
+    pp.Ng = pp.p.Nss / pp.na_buf;
+    pp.fs_chunk = fs_chunk_for(d, pp, 1);
+    pp.hold_k = 0;
+    if (pp.p.Nis > 1) {
+      const size_t per_buf = (size_t)cfg->Nc * pp.p.Nss * N * cfg->Np * sizeof(float2);
+      size_t k = kHoldBytes / per_buf;
+      k = k > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : k;
+      k = k > pp.p.Nis ? pp.p.Nis : k;
+      pp.hold_k = k >= 2 ? (unsigned)k : 0;
+    }
+    // batches of consecutive buffers only where an integration spans
+    // several buffers (then each dump group is the whole buffer)
+    pp.fs_bmax = 1;
+    if (pp.p.Nis > 1) {
+      const size_t per_buf = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
+      size_t bm = pp.fs_budget / per_buf;
+      bm = bm < 1 ? 1 : bm;
+      bm = bm > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : bm;
+      pp.fs_bmax = (unsigned)(bm > pp.p.Nis ? pp.p.Nis : bm);
+    }
+    for (unsigned kb = 1; kb <= pp.fs_bmax; ++kb) {
+      const size_t wl = (size_t)fs_chunk_for(d, pp, kb) * kb * pp.p.Nss * N * cfg->Np;
+      if (wl > work) work = wl;
+    }
+  }
+  return work;
 }
 
 // Sizes and launch plans for a configuration, without touching the device.
@@ -345,39 +396,12 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
     }
     pp.Q = pp.p.npolout == 1 ? 1 : 4;
     pp.small = N <= (unsigned)rsk::kTwN;
-    if (pp.small) {
-      plan_small(d, pp);
-      plan_small_mfma(d, pp);
-    } else {
-      const unsigned L = ilog2u(N);
-      pp.N1 = 1u << ((L + 1) / 2);
-      pp.N2 = 1u << (L / 2);
-      pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
-      pp.Ng = pp.p.Nss / pp.na_buf;
-      pp.fs_chunk = fs_chunk_for(d, pp, 1);
-      pp.hold_k = 0;
-      if (pp.p.Nis > 1) {
-        const size_t per_buf = (size_t)cfg->Nc * pp.p.Nss * N * cfg->Np * sizeof(float2);
-        size_t k = kHoldBytes / per_buf;
-        k = k > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : k;
-        k = k > pp.p.Nis ? pp.p.Nis : k;
-        pp.hold_k = k >= 2 ? (unsigned)k : 0;
-      }
-      // batches of consecutive buffers only where an integration spans
-      // several buffers (then each dump group is the whole buffer)
-      pp.fs_bmax = 1;
-      if (pp.p.Nis > 1) {
-        const size_t per_buf = (size_t)pp.p.Nss * N * cfg->Np * sizeof(float2);
-        size_t bm = kFsWorkBytes / per_buf;
-        bm = bm < 1 ? 1 : bm;
-        bm = bm > (size_t)rsk::kMaxBatch ? (size_t)rsk::kMaxBatch : bm;
-        pp.fs_bmax = (unsigned)(bm > pp.p.Nis ? pp.p.Nis : bm);
-      }
-      for (unsigned kb = 1; kb <= pp.fs_bmax; ++kb) {
-        const size_t wl = (size_t)fs_chunk_for(d, pp, kb) * kb * pp.p.Nss * N * cfg->Np;
-        if (wl > work_max) work_max = wl;
-      }
-    }
+    pp.t_target = 2048;
+    pp.t_maxspg = 16;
+    pp.t_mtarget = 2048;
+    pp.fs_budget = kFsWorkBytes;
+    const size_t wl = plan_product(d, pp);
+    if (wl > work_max) work_max = wl;
     const size_t pn = part_need(d, pp);
     if (pn > part_max) part_max = pn;
     pp.acc_len = (size_t)pp.Q * pp.p.Nds * cfg->Nc * N;
@@ -774,6 +798,92 @@ int rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n
   if (first) pp.holding = 0;
   return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, isign, gam,
                       1.0f / (full * full), first, last);
+}
+
+// ---- autotuning of the launch plans -------------------------------------
+// Grows the partial-sum and intermediate buffers to what product i's
+// current plan needs (the device is idle while tuning).
+static int fit_buffers(rsd_ctx *d, ProdPlan &pp, size_t work) {
+  const size_t pn = part_need(d, pp);
+  if (pn > d->part_len) {
+    if (d->d_part) RSD_CHECK(hipFree(d->d_part));
+    d->d_part = nullptr;
+    if (alloc_zero((void **)&d->d_part, pn * sizeof(float))) return 1;
+    d->part_len = pn;
+  }
+  if (work > d->work_len) {
+    if (d->d_work) RSD_CHECK(hipFree(d->d_work));
+    d->d_work = nullptr;
+    if (alloc_zero((void **)&d->d_work, work * sizeof(float2))) return 1;
+    d->work_len = work;
+  }
+  return 0;
+}
+
+namespace {
+struct Tunables { unsigned target, maxspg, mtarget; size_t budget; };
+}
+
+static void set_tunables(ProdPlan &pp, const Tunables &t) {
+  pp.t_target = t.target;
+  pp.t_maxspg = t.maxspg;
+  pp.t_mtarget = t.mtarget;
+  pp.fs_budget = t.budget;
+}
+
+// Candidate tunables of product i, the defaults first.  Small products: the
+// workgroups a launch aims for and the slots per dump group (on the
+// matrix-core engine, the wavefronts a launch aims for); four-step products:
+// the intermediate's budget per launch pair.
+static std::vector<Tunables> tune_cands(const rsd_ctx *d, const ProdPlan &pp) {
+  const Tunables dflt = {2048u, 16u, 2048u, kFsWorkBytes};
+  std::vector<Tunables> c = {dflt};
+  if (pp.small && d->mfma && pp.m_ok) {
+    for (unsigned tg : {512u, 1024u, 4096u, 8192u}) c.push_back({dflt.target, dflt.maxspg, tg, dflt.budget});
+  } else if (pp.small) {
+    for (unsigned tg : {512u, 1024u, 2048u, 4096u, 8192u})
+      for (unsigned sg : {2u, 4u, 8u, 16u, 32u})
+        c.push_back({tg, sg, dflt.mtarget, dflt.budget});
+  } else {
+    for (size_t mb : {256, 512, 1024, 2048, 3072}) c.push_back({dflt.target, dflt.maxspg, dflt.mtarget, mb << 20});
+  }
+  return c;
+}
+
+// The launch plan the current tunables produce, for skipping duplicates.
+static std::vector<unsigned long long> plan_key(const ProdPlan &pp) {
+  if (pp.small) return {pp.chunk, pp.G, pp.spg, pp.gpw, pp.m_chunk};
+  return {pp.fs_chunk, pp.fs_bmax};
+}
+
+int rsd_tune_count(rsd_ctx *d, int i) { return (int)tune_cands(d, d->pp[i]).size(); }
+
+int rsd_tune_apply(rsd_ctx *d, int i, int k) {
+  ProdPlan &pp = d->pp[i];
+  const std::vector<Tunables> c = tune_cands(d, pp);
+  if (k < 0 || k >= (int)c.size()) return 1;
+  // a candidate whose plan equals an earlier candidate's is a duplicate
+  std::vector<std::vector<unsigned long long>> seen;
+  for (int j = 0; j < k; ++j) {
+    set_tunables(pp, c[j]);
+    (void)plan_product(d, pp);
+    seen.push_back(plan_key(pp));
+  }
+  set_tunables(pp, c[k]);
+  const size_t work = plan_product(d, pp);
+  if (std::find(seen.begin(), seen.end(), plan_key(pp)) != seen.end()) return 2;
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  return fit_buffers(d, pp, work);
+}
+
+int rsd_tune_describe(rsd_ctx *d, int i, char *buf, size_t len) {
+  const ProdPlan &pp = d->pp[i];
+  if (pp.small)
+    return snprintf(buf, len, "small target %u spg %u mtarget %u (chunk %u, %u per group, %u groups per workgroup)",
+                    pp.t_target, pp.t_maxspg, pp.t_mtarget, pp.chunk, pp.spg, pp.gpw);
+  return snprintf(buf, len, "fourstep budget %zu MiB (chunk %u channels, batch %u)", pp.fs_budget >> 20,
+                  pp.fs_chunk, pp.fs_bmax);
 }
 
 int rsd_lanes_begin(rsd_ctx *d) {

@@ -264,3 +264,82 @@ def run_device_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, b
     assert all(seen[i] == len(got[i]) for i in range(No)), (seen, {i: len(v) for i, v in got.items()})
     del dev
     return results, cfg
+
+
+def run_calls(pattern, *, Nc, Ntpb, Nts, Nas, Npolout, held=None, mfma=False, tone=None,
+                fft_dir=1, conj=0, seed=1):
+    """pattern: a list of calls, each k > 0 (k buffers: 1 through
+    rawspec_amd_start_processing_device, more through the batch call),
+    "reset" (rawspec_reset_integration) or "autotune" / "autotune:<k>"
+    (rawspec_amd_autotune on the next buffer for k buffers per call, which
+    restarts the integrations the same way).
+    held: rawspec_amd_set_held_buffers before the first call.  Returns
+    (results as in run_case, cfg, the GPU's dumps per product in order,
+    the autotune reports)."""
+    import torch
+    import rawspec_amd as R
+
+    print(f"case: {pattern} held={held} mfma={mfma} Nts={Nts} Nas={Nas}", flush=True)
+    No = len(Nts)
+    got = {i: [] for i in range(No)}
+
+    def on_dump(sp, product, cbtype):
+        if cbtype == R.RAWSPEC_CALLBACK_POST_DUMP:
+            got[product].append(sp.pwrbuf(product).copy())
+
+    sp = R.Spectrometer(Nc=Nc, Ntpb=Ntpb, Nts=Nts, Nas=Nas, Np=2, Nbps=8, Npolout=Npolout,
+                        input_conjugated=conj, callback=on_dump)
+    assert sp.get_held_buffers() == 8
+    if held is not None:
+        assert sp.set_held_buffers(held) == 0 and sp.get_held_buffers() == held
+    if mfma:
+        assert sp.set_mfma(True) == 0
+    cfg = O.make_cfg(No=No, Np=2, Nant=1, Nc=Nc, Ntpb=Ntpb, Nbps=8, Npolout=Npolout, Nts=Nts,
+                     Nas=Nas, input_conjugated=conj)
+    assert O.derive(cfg) == 0
+    nbuf = sum(k for k in pattern if isinstance(k, int))
+    Nb_ = sp.ctx.Nb
+    host = [O.device_layout(gen_blocks(Nc, Ntpb, 2, 8, Nb_, seed=seed, tone=tone,
+                                       first_blk=n * Nb_), Nc, Ntpb, 2) for n in range(nbuf)]
+    dev = [torch.from_numpy(h.view(np.uint8).copy()).to("cuda") for h in host]
+    torch.cuda.synchronize()
+    st = O.State(cfg)
+
+    def f32_refs():
+        return {i: Fp32Ref(cfg, i) for i in range(No) if Nts[i] >= FP32_FLOOR_NT}
+
+    f32 = f32_refs()
+    refs = {i: [] for i in range(No)}
+    reports = []
+    n = 0
+    for k in pattern:
+        if isinstance(k, str):
+            if k == "reset":
+                assert sp.reset_integration() == 0
+            else:
+                per_call = int(k.split(":")[1]) if ":" in k else 1
+                reports.append(sp.autotune(dev[min(n, nbuf - 1)].data_ptr(), per_call))
+            st.reset()
+            f32 = f32_refs()
+            continue
+        if k == 1:
+            assert sp.start_processing_device(dev[n].data_ptr(), fft_dir) == 0
+        else:
+            assert sp.start_processing_device_batch([d.data_ptr() for d in dev[n:n + k]],
+                                                    fft_dir) == 0
+        for h in host[n:n + k]:
+            _, ref, _ = st.process(h, fft_dir)
+            f32out = {i: r.process(h, fft_dir) for i, r in f32.items()}
+            for i in ref:
+                refs[i].append((ref[i], f32out.get(i)))
+        n += k
+    assert sp.wait_for_completion() == 0
+    sp.close()
+    st.close()
+    results = []
+    for i in range(No):
+        assert len(got[i]) == len(refs[i]), (i, len(got[i]), len(refs[i]))
+        for g, (r, f) in zip(got[i], refs[i]):
+            results.append((0, i, g, r, None, None, f))
+    del dev
+    return results, cfg, got, reports
