@@ -27,7 +27,9 @@ kernel, in which the products run one after another.
 Also measured, reported under "workloads": the other single-stream
 workloads (8 channels with 4 MiB buffers and Nfft 1024; the headline geometry
 with pols 1 and pols 4; 16 antennas x 16 channels of complex int16 with four
-products and the incoherent sum) and, under "serving", the
+products and the incoherent sum), one buffer per call, 8 per call, and 8 per
+call after rawspec_amd_autotune picked launch plans for that mode; under
+"serving", the
 serving loop (32 Poisson-arriving streams of random shapes, concurrent vs
 one at a time); under "file_ingest", the rawspec program reading a GUPPI RAW
 file of the headline geometry from the page cache and writing filterbank
@@ -216,6 +218,13 @@ def other_config(torch, R, W, nring=8):
     dt8, _ = timed(torch, sp, ring, sb, warmup=8, batch=8)
     out["served_GBps"] = round(sb * buf_bytes / dt8 / 1e9, 2)
     out["served_ms_per_step"] = round(dt8 / sb * 1e3, 4)
+    # the launch-plan autotuner for batches of 8, then the batches again
+    t1 = time.perf_counter()
+    rep = sp.autotune(ring[0].data_ptr(), 8)
+    out["autotune_s"] = round(time.perf_counter() - t1, 2)
+    dt8t, _ = timed(torch, sp, ring, sb, warmup=8, batch=8)
+    out["served_autotuned_GBps"] = round(sb * buf_bytes / dt8t / 1e9, 2)
+    out["autotune_plans"] = [ln.split(":")[0] for ln in rep.strip().splitlines()]
     sp.close()
     del ring
     torch.cuda.empty_cache()
