@@ -138,9 +138,13 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   }
 }
 
-// at most 256 registers per thread, so two waves share each SIMD
+// Waves per SIMD pass B's register budget allows (2: at most 256 registers
+// per thread).
+#ifndef RSK_FSB_WAVES
+#define RSK_FSB_WAVES 2
+#endif
 template <int N1, int NP, int PMODE>
-__global__ __launch_bounds__(wg_team(N1)) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(wg_team(N1)) __attribute__((amdgpu_waves_per_eu(RSK_FSB_WAVES)))
 void k_fs_b(FsArgs a) {
   constexpr int E = elems_fs(N1);
   constexpr int T = N1 / E;
