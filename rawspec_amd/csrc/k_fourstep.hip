@@ -90,51 +90,79 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   const unsigned N1 = N1C;
   const unsigned long long N = (unsigned long long)N1 * N2;
   const unsigned tiles = N1 / TA;
-  const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
-  const unsigned tile = l % tiles;
-  const unsigned cs = l / tiles;
-  const unsigned sp = cs % a.Nss;      // spectrum within its buffer
-  const unsigned cb = cs / a.Nss;
-  const unsigned b = cb % a.nbat;      // buffer within the batch
-  const unsigned cl = cb / a.nbat;     // channel within the chunk
   const unsigned nsp = a.Nss * a.ynb;  // spectra per channel in Y
+  const unsigned items = tiles * a.Nss * a.nbat * a.nch;
   const int tid = threadIdx.x;
-  const int sg = tid % TA;             // n1 slot: fastest across lanes
-  const int j = tid / TA;
-  const unsigned n1 = tile * TA + sg;
-
-  const W *src = reinterpret_cast<const W *>(a.in[b] + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
-                 (unsigned long long)sp * N + n1;
-  float2 v[NP][E];
-  W raw[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) raw[m] = src[(unsigned long long)(j + m * T) * N1];
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    float2 o[NP];
-    S::cvt(raw[m], o, a.isign);
-#pragma unroll
-    for (int p = 0; p < NP; ++p) v[p][m] = o[p];
-  }
-  team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
-
-  // Y[cl][(yb0 + b) * Nss + sp][k2 / TBY][n1][k2 % TBY][pol]; offsets within
-  // one spectrum (N <= 2^24 values) fit 32 bits
+  int sg = tid % TA;                   // n1 slot: fastest across lanes
+  int j = tid / TA;
   constexpr unsigned TBY = YF == Y_ROWS ? 1 : tile_of(N1C);
-  const unsigned long long spec = (unsigned long long)cl * nsp + (a.yb0 + b) * a.Nss + sp;
-  float2 *dst = a.work + spec * N * NP;
+
+  // Item l: (channel cl, buffer b, spectrum sp, tile).  The workgroup takes
+  // items l0, l0 + G, l0 + 2G, ... (G workgroups), loading the next item's
+  // samples before transforming the current one, so the loads overlap the
+  // transform; workgroup ids are remapped so each XCD takes a contiguous
+  // range of every round of G items.
+  auto load = [&](unsigned l, W (&raw)[E]) {
+    const unsigned tile = l % tiles;
+    const unsigned cs = l / tiles;
+    const unsigned sp = cs % a.Nss;    // spectrum within its buffer
+    const unsigned cb = cs / a.Nss;
+    const unsigned b = cb % a.nbat;    // buffer within the batch
+    const unsigned cl = cb / a.nbat;   // channel within the chunk
+    const W *src = reinterpret_cast<const W *>(a.in[b] + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
+                   (unsigned long long)sp * N + tile * TA + sg;
 #pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const unsigned k2 = j + m * T;
-    const unsigned e = n1 * k2;            // < N <= 2^24
-    const float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
-    const unsigned off = y_off<T, TBY, N1C>(j, n1, m);
-    if constexpr (NP == 2) {
-      const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
-      reinterpret_cast<float4 *>(dst)[off] = make_float4(y0.x, y0.y, y1.x, y1.y);
-    } else {
-      dst[off] = cmul(v[0][m], w);
+    for (int m = 0; m < E; ++m) raw[m] = src[(unsigned long long)(j + m * T) * N1];
+  };
+  unsigned l = xcd_remap(blockIdx.x, gridDim.x);
+  W raw[E];
+  load(l, raw);
+  for (;;) {
+    // j and sg pass through an empty asm each item, so the compiler
+    // recomputes the item's addresses instead of keeping 16 loads' and 16
+    // stores' offsets live across the loop (which cost a wave per SIMD)
+    asm volatile("" : "+v"(j), "+v"(sg));
+    const unsigned ln = l + gridDim.x;   // uniform across the workgroup
+    W rawn[E];
+    if (ln < items) load(ln, rawn);
+    float2 v[NP][E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      float2 o[NP];
+      S::cvt(raw[m], o, a.isign);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
+    team_fft<N2, E, NP, TwTab<N2>::TWN>(v, lds + sg * team_stride<N2, NP>(), j, twp);
+
+    // Y[cl][(yb0 + b) * Nss + sp][k2 / TBY][n1][k2 % TBY][pol]; offsets
+    // within one spectrum (N <= 2^24 values) fit 32 bits
+    const unsigned tile = l % tiles;
+    const unsigned cs = l / tiles;
+    const unsigned sp = cs % a.Nss;
+    const unsigned cb = cs / a.Nss;
+    const unsigned b = cb % a.nbat;
+    const unsigned cl = cb / a.nbat;
+    const unsigned n1 = tile * TA + sg;
+    const unsigned long long spec = (unsigned long long)cl * nsp + (a.yb0 + b) * a.Nss + sp;
+    float2 *dst = a.work + spec * N * NP;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const unsigned k2 = j + m * T;
+      const unsigned e = n1 * k2;          // < N <= 2^24
+      const float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
+      const unsigned off = y_off<T, TBY, N1C>(j, n1, m);
+      if constexpr (NP == 2) {
+        const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
+        reinterpret_cast<float4 *>(dst)[off] = make_float4(y0.x, y0.y, y1.x, y1.y);
+      } else {
+        dst[off] = cmul(v[0][m], w);
+      }
+    }
+    if (ln >= items) break;
+    l = ln;
+#pragma unroll
+    for (int m = 0; m < E; ++m) raw[m] = rawn[m];
   }
 }
 
@@ -279,10 +307,11 @@ static int go_a_n(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
 
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
-  const unsigned long long nb = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
+  const unsigned long long items = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
+  const unsigned long long nb = a.grid_a && a.grid_a < items ? a.grid_a : items;
   // Y's k2 tile: the vector pass B's, or 1 for the matrix-core pass B
   const bool tiled = a.tby == (unsigned)fs_tile_b((int)a.N1);
-  if (nb == 0 || nb > 0x7fffffffull || !(tiled || a.tby == 1) || a.nbat == 0 ||
+  if (nb == 0 || items > 0x7fffffffull || !(tiled || a.tby == 1) || a.nbat == 0 ||
       a.nbat > (unsigned)kMaxBatch || a.yb0 + a.nbat > a.ynb)
     return 2;
   if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);

@@ -73,6 +73,7 @@ struct FsArgs {
   // writes this launch's nbat buffers to slots yb0 .., pass B reads slots
   // 0 .. nbat - 1
   unsigned yb0, ynb;
+  unsigned grid_a;                // pass-A workgroups, each looping over items (0: one per item)
   unsigned tby;                   // k2 tile of Y: fs_tile_b(N1) for k_fs_b, 1 for k_fs_b_mfma
   float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
 };

@@ -96,6 +96,11 @@ struct ProdPlan {
 constexpr size_t kFsWorkBytes = (size_t)RSD_FS_WORK_MB << 20;
 // smallest input buffer for which a context runs its products on two lanes
 constexpr size_t kLaneMinBytes = (size_t)64 << 20;
+// Pass-A workgroups per CU: each loops over its share of the launch's items
+// (0: one workgroup per item).
+#ifndef RSD_FSA_WG_PER_CU
+#define RSD_FSA_WG_PER_CU 8
+#endif
 // Held first-pass output per context, at most (one-buffer calls): 8 buffers
 // of a 256 MiB, 2^20-point product
 #ifndef RSD_HOLD_MB
@@ -141,6 +146,7 @@ struct rsd_ctx {
   size_t part_len;
   float2 *d_work;
   size_t work_len;
+  unsigned cus;                        // compute units of the device
   unsigned hold_max;                   // buffers a product may hold (rsd_set_hold)
   size_t held_bytes;                   // allocated for held output
   float2 *d_tw;
@@ -431,6 +437,11 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
   if (plan_all(cfg, d)) return 1;
   if (cfg->ics) d->weights.assign(cfg->weights, cfg->weights + cfg->Nant);
   RSD_CHECK(hipSetDevice(d->dev));
+  {
+    int cus = 0;
+    RSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->dev));
+    d->cus = cus > 0 ? (unsigned)cus : 1u;
+  }
   if (alloc_zero((void **)&d->d_in, d->in_bytes)) return 1;
   if (cfg->expand4 && alloc_zero((void **)&d->d_stage, (size_t)(d->chan_block_bytes / 2) * cfg->Nc))
     return 1;
@@ -485,6 +496,7 @@ static rsd_ctx *new_ctx() {
   d->d_part = nullptr;
   d->d_work = d->d_tw = nullptr;
   d->hold_max = rsk::kMaxBatch;
+  d->cus = 1;
   d->held_bytes = 0;
   d->d_w = nullptr;
   d->prof = false;
@@ -655,6 +667,7 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
     a.mscale = (float)ldexp(1.0, -e);
   }
   a.yb0 = 0;
+  a.grid_a = d->cus * RSD_FSA_WG_PER_CU;
 }
 
 static int launch_pass_b(rsd_ctx *d, int i, const rsk::FsArgs &a, hipStream_t s) {
