@@ -24,16 +24,26 @@ constexpr int wg_for(int n) {
   return n <= 16 ? 64 : (n / elems_for(n) > 256 ? n / elems_for(n) : 256);
 }
 
+// LDS of one workgroup (float2): the FFT exchange area, reused for the sums
+// of half the slots (a group of one slot writes from registers), and the
+// twiddle table.
+template <int N, int NP, int PMODE>
+constexpr int small_lds_f2() {
+  constexpr int E = elems_for(N), T = N / E, SLOTS = wg_for(N) / T, Q = nq<PMODE>();
+  constexpr int FFT_F2 = (N > E) ? SLOTS * team_stride<N, NP>() : 1;
+  constexpr int RED_F2 = SLOTS >= 2 ? ((SLOTS / 2) * Q * N + 1) / 2 : 1;
+  return FFT_F2 > RED_F2 ? FFT_F2 : RED_F2;
+}
+// at most 256 registers per thread, so two waves share each SIMD
 template <int N, int NBPS, int NP, int PMODE>
-__global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
+__global__ __launch_bounds__(wg_for(N)) __attribute__((amdgpu_waves_per_eu(2)))
+void k_small(SmallArgs a) {
   constexpr int E = elems_for(N);
   constexpr int T = N / E;
   constexpr int WG = wg_for(N);
   constexpr int SLOTS = WG / T;
   constexpr int Q = nq<PMODE>();
-  constexpr int FFT_F2 = (N > E) ? SLOTS * team_stride<N, NP>() : 1;
-  constexpr int RED_F2 = (SLOTS * Q * N + 1) / 2;
-  constexpr int LDS_F2 = FFT_F2 > RED_F2 ? FFT_F2 : RED_F2;
+  constexpr int LDS_F2 = small_lds_f2<N, NP, PMODE>();
   using S = Sample<NBPS, NP>;
   using W = typename S::word;
   __shared__ float2 lds[LDS_F2];
@@ -103,14 +113,55 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
     }
   }
 
-  // Every slot parks its sums in LDS ([slot][q][bin]); the write-out loop
-  // adds the slots of one group in increasing slot order.
   __syncthreads();   // the FFT exchange area is reused
+  float *base = a.direct ? a.acc : a.part;
+  const bool accum = a.direct && !a.first;
+  if (a.spg == 1) {
+    // Every slot is a group of its own: its sums go straight from registers
+    // (consecutive lanes hold consecutive bins).
+    const unsigned gg = g0 + slot;
+    if (gg >= a.Ng) return;
+    float *bg = a.direct ? base + (((unsigned long long)gg * Q) * a.Nc + c) * N
+                         : base + ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q) * N;
+    const unsigned long long qstride = a.direct ? (unsigned long long)a.Nc * N : N;
+    const float sc = a.direct ? a.scale : 1.0f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      float *bq = bg + q * qstride;
+      float val[E];
+#pragma unroll
+      for (int m = 0; m < E; ++m) val[m] = acc[m][q] * sc;
+      if (accum) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) val[m] += bq[(j + m * T + N / 2) & (N - 1)];
+      }
+#pragma unroll
+      for (int m = 0; m < E; ++m) bq[(j + m * T + N / 2) & (N - 1)] = val[m];   // bin j + m*T
+    }
+    return;
+  }
+  // Groups of spg >= 2 slots: the upper half of each group's slots parks its
+  // sums in LDS ([group][slot - half][q][bin]), the lower half adds its own
+  // to them, and the write-out loop adds the half slots of one group in
+  // increasing order, so the order of the sums is fixed.
   float *red = reinterpret_cast<float *>(lds);
+  const int half = (int)a.spg >> 1;
+  if (si >= half) {
 #pragma unroll
-  for (int m = 0; m < E; ++m)
+    for (int m = 0; m < E; ++m)
 #pragma unroll
-    for (int q = 0; q < Q; ++q) red[(slot * Q + q) * N + j + m * T] = acc[m][q];
+      for (int q = 0; q < Q; ++q) red[((gi * half + si - half) * Q + q) * N + j + m * T] = acc[m][q];
+  }
+  __syncthreads();
+  if (si < half) {
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        float *r = red + ((gi * half + si) * Q + q) * N + j + m * T;
+        *r = acc[m][q] + *r;
+      }
+  }
   __syncthreads();
 
   // Write-out in batches of OB outputs per thread: sums first, then (when
@@ -118,8 +169,6 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
   // stores, so read-modify-writes do not serialise on memory latency.
   constexpr int OB = 4;
   const int nout = a.gpw * Q * N;
-  float *base = a.direct ? a.acc : a.part;
-  const bool accum = a.direct && !a.first;
   for (int i0 = tid; i0 < nout; i0 += OB * WG) {
     float val[OB];
     unsigned long long dst[OB];
@@ -135,9 +184,9 @@ __global__ __launch_bounds__(wg_for(N)) void k_small(SmallArgs a) {
       dst[u] = ~0ull;
       val[u] = 0.0f;
       if (idx < nout && gg < a.Ng) {
-        const float *rp = red + ((go * a.spg) * Q + q) * N + k;
+        const float *rp = red + ((go * half) * Q + q) * N + k;
         float sum = rp[0];
-        for (int s2 = 1; s2 < a.spg; ++s2) sum += rp[s2 * Q * N];
+        for (int s2 = 1; s2 < half; ++s2) sum += rp[s2 * Q * N];
         if (a.direct) {
           dst[u] = (((unsigned long long)gg * Q + q) * a.Nc + c) * N + jo;
           val[u] = sum * a.scale;
