@@ -58,10 +58,6 @@ __device__ __forceinline__ unsigned y_off(unsigned j, unsigned n1, int m) {
     return ((k2 / TBY) * N1C + n1) * TBY + (k2 % TBY);
   }
 }
-// Y layouts: tiled for the vector pass B (k2 tile tile_of(N1)), plain
-// [k2][n1] rows for the matrix-core pass B.
-enum { Y_TILED = 0, Y_ROWS = 1 };
-
 // Pass A for N = N1 * N2 with N1 = R1 * N2 (R1 = 1 or 2).  The four-step
 // twiddle W_N^e, e = n1 * k2 < N, is split as e = h * N1 + r:
 // W_N^(h * N1) = W_N2^h is the sub-transform's own table (already in LDS)
@@ -69,7 +65,7 @@ enum { Y_TILED = 0, Y_ROWS = 1 };
 // N1 entries of the FP64-generated fine table; so each twiddle costs two LDS
 // reads and one complex product (two correctly rounded factors, as with the
 // global two-level table) instead of two scattered global gathers.
-template <int N2, int R1, int NBPS, int NP, int YF>
+template <int N2, int R1, int NBPS, int NP>
 __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   constexpr int E = elems_fs(N2);
   constexpr int T = N2 / E;
@@ -95,7 +91,7 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
   const int tid = threadIdx.x;
   int sg = tid % TA;                   // n1 slot: fastest across lanes
   int j = tid / TA;
-  constexpr unsigned TBY = YF == Y_ROWS ? 1 : tile_of(N1C);
+  constexpr unsigned TBY = tile_of(N1C);  // k2 tile of Y: pass B's workgroup tile
 
   // Item l: (channel cl, buffer b, spectrum sp, tile).  The workgroup takes
   // items l0, l0 + G, l0 + 2G, ... (G workgroups), loading the next item's
@@ -268,17 +264,11 @@ void k_fs_b(FsArgs a) {
 int fs_tile_a(int n2) { return (int)(wg_team(n2) / (n2 / elems_fs(n2))); }
 int fs_tile_b(int n1) { return (int)(wg_team(n1) / (n1 / elems_fs(n1))); }
 
-template <int N2, int R1, int NBPS, int YF>
-static void go_a_y(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
-  constexpr int WGT = wg_team(N2);
-  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2, YF>), dim3(nb), dim3(WGT), 0, s, a);
-  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1, YF>), dim3(nb), dim3(WGT), 0, s, a);
-}
-
 template <int N2, int R1, int NBPS>
 static int go_a_r(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
-  if (a.tby == 1) go_a_y<N2, R1, NBPS, Y_ROWS>(np, a, nb, s);
-  else go_a_y<N2, R1, NBPS, Y_TILED>(np, a, nb, s);
+  constexpr int WGT = wg_team(N2);
+  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a);
+  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -309,9 +299,8 @@ int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
   const unsigned long long items = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
   const unsigned long long nb = a.grid_a && a.grid_a < items ? a.grid_a : items;
-  // Y's k2 tile: the vector pass B's, or 1 for the matrix-core pass B
-  const bool tiled = a.tby == (unsigned)fs_tile_b((int)a.N1);
-  if (nb == 0 || items > 0x7fffffffull || !(tiled || a.tby == 1) || a.nbat == 0 ||
+  // Y's k2 tile: pass B's workgroup tile
+  if (nb == 0 || items > 0x7fffffffull || a.tby != (unsigned)fs_tile_b((int)a.N1) || a.nbat == 0 ||
       a.nbat > (unsigned)kMaxBatch || a.yb0 + a.nbat > a.ynb)
     return 2;
   if (nbps == 8) return go_a_n<8>(np, a, (unsigned)nb, s);

@@ -33,7 +33,7 @@ __device__ __forceinline__ void fs_b_column(const float4 *y, float s1, float gam
 #pragma unroll
   for (int kb = 0; kb < 4; ++kb)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) v[kb][i] = y[col + 32 * (8 * kb + 4 * h + i)];
+    for (int i = 0; i < 4; ++i) v[kb][i] = y[kMfmaWaves * (col + 32 * (8 * kb + 4 * h + i))];
   f16v xr[2], xi[2];
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
@@ -94,9 +94,11 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void
 
   float acc[16][Q];
   const float s1 = a.mscale;
+  // Y tiled [k2 / 4][n1][k2 % 4][pol]: the workgroup's 4 columns are one
+  // tile, interleaved, so this wavefront's column is every 4th float4
   const float4 *y0 = reinterpret_cast<const float4 *>(a.work) +
                      ((unsigned long long)cl * a.Nss * a.ynb + g * a.na_buf) * N +
-                     (unsigned long long)k2 * N1;
+                     (unsigned long long)tile * N1 * kMfmaWaves + wv;
   if constexpr (ONE) {
     fs_b_column<PMODE, true>(y0, s1, a.gam, K, acc);
   } else {
@@ -299,7 +301,7 @@ int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
 
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {
   if (!fs_b_mfma_ok((int)a.N1, np) || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f) ||
-      a.nbat == 0 || a.nbat > a.ynb)
+      a.nbat == 0 || a.nbat > a.ynb || a.tby != (unsigned)kMfmaWaves)
     return 2;
   const unsigned long long nb = (unsigned long long)(a.N2 / kMfmaWaves) * a.Ng * a.nch;
   if (nb == 0 || nb > 0x7fffffffull) return 2;

@@ -74,7 +74,7 @@ struct FsArgs {
   // 0 .. nbat - 1
   unsigned yb0, ynb;
   unsigned grid_a;                // pass-A workgroups, each looping over items (0: one per item)
-  unsigned tby;                   // k2 tile of Y: fs_tile_b(N1) for k_fs_b, 1 for k_fs_b_mfma
+  unsigned tby;                   // k2 tile of Y: fs_tile_b(N1), both pass-B kernels
   float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
 };
 
@@ -110,7 +110,8 @@ int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
 int small_mfma_ok(int n, int np);
 int launch_small_mfma(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
                       hipStream_t s);
-// Matrix-core pass B (N1 = 1024, two pols); needs Y written with tby = 1.
+// Matrix-core pass B (N1 = 1024, two pols): a workgroup's 4 columns are one
+// k2 tile of Y (tby = fs_tile_b(1024) = 4).
 int fs_b_mfma_ok(int n1, int np);
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s);
 int launch_reduce(const ReduceArgs &a, hipStream_t s);

@@ -659,7 +659,9 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   // completes (the two pass-B kernels lay out the internal accumulator
   // differently).
   if (first) pp.fs_mfma = d->mfma && rsk::fs_b_mfma_ok((int)pp.N1, (int)c.Np);
-  a.tby = pp.fs_mfma ? 1u : (unsigned)rsk::fs_tile_b((int)pp.N1);
+  // both pass-B kernels read Y tiled by fs_tile_b(N1) = 4 consecutive k2
+  // (for N1 = 1024, the matrix-core pass B's workgroup of 4 columns)
+  a.tby = (unsigned)rsk::fs_tile_b((int)pp.N1);
   {
     // largest |Y| is N2 * full * sqrt(2); scale it to at most 2^15
     int e = 0;
