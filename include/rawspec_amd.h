@@ -22,9 +22,10 @@ extern "C" {
 /* Validates and derives exactly as rawspec_initialize() does (updating the
  * same fields of ctx) but allocates nothing; reports the device bytes and the
  * pinned host bytes rawspec_initialize() would allocate.  For admission
- * control by HBM budget.  Held first-pass output
- * (rawspec_amd_set_held_buffers) is allocated later, at first use, and not
- * included.  Returns 0 when the configuration is valid. */
+ * control by HBM budget.  The device bytes include the held first-pass
+ * output (rawspec_amd_set_held_buffers) the context may allocate at first
+ * use with the default setting; a caller that turns holding off gets an
+ * upper bound.  Returns 0 when the configuration is valid. */
 int rawspec_amd_plan(rawspec_context *ctx, size_t *device_bytes, size_t *host_bytes);
 
 /* Device pointer and size of the context's input buffer, laid out

@@ -422,15 +422,20 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
 static size_t plan_bytes(const rsd_ctx *d) {
   size_t b = d->in_bytes + (d->cfg.expand4 ? (size_t)(d->chan_block_bytes / 2) * d->cfg.Nc : 0);
   b += (size_t)rsk::kTwN * sizeof(float2) + (d->cfg.ics ? d->cfg.Nant * sizeof(float) : 0);
+  size_t held = 0;
   for (unsigned i = 0; i < d->cfg.No; ++i) {
     const ProdPlan &pp = d->pp[i];
     b += pp.acc_len * sizeof(float) + pp.ics_len * sizeof(float);
     if (!pp.small) {
       b += (size_t)pp.N1 * sizeof(float2);
       if (pp.p.Nis > 1) b += pp.acc_len * sizeof(float);
+      // held first-pass output, allocated at first use under the same cap
+      // (hold_ready), with the default number of held buffers
+      const size_t h = (size_t)pp.hold_k * d->cfg.Nc * pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
+      if (pp.hold_k >= 2 && held + h <= kHoldBytes) held += h;
     }
   }
-  return b + d->part_len * sizeof(float) + d->work_len * sizeof(float2);
+  return b + held + d->part_len * sizeof(float) + d->work_len * sizeof(float2);
 }
 
 static int create_impl(const rsd_config *cfg, rsd_ctx *d) {

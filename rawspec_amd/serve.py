@@ -193,10 +193,6 @@ class _Stream:
         # streams each would share the few hardware queues badly
         if self.sp.set_lanes(srv.lanes):
             raise RuntimeError("rawspec_amd_set_lanes failed")
-        # the loop batches queued buffers itself; held first-pass output would
-        # add up to 8 GiB per stream to the admitted HBM budget
-        if self.sp.set_held_buffers(0):
-            raise RuntimeError("rawspec_amd_set_held_buffers failed")
         assert self.sp.device_input_size() == plan_.buf_bytes
         self.hs = torch.cuda.ExternalStream(self.sp.stream(), device=f"cuda:{srv.gpu}")
         self.ring = srv.make_ring(req, plan_)

@@ -173,8 +173,10 @@ def test_plan_bytes_for_the_64_channel_three_product_workload(rlib):
     inter-buffer accumulator since it integrates over 51 buffers, 8 MiB for
     Nfft 8 with 1024 spectra per dump, 1 MiB for 1024), the 1536 MiB
     four-step intermediate (all 64 channels of one buffer, or batches of 8
-    buffers x 12 channels), and twiddle tables; the rest is the small
-    path's partial sums, bounded by one more integration buffer set."""
+    buffers x 12 channels), the 8 GiB store of first-pass output the 2^20
+    product holds across one-buffer calls (8 buffers x 1 GiB), and twiddle
+    tables; the rest is the small path's partial sums, bounded by one more
+    integration buffer set."""
     c = rlib.RawspecContext()
     c.No, c.Np, c.Nc, c.Ntpb, c.Nbps, c.Nb = 3, 2, 64, 1 << 19, 8, 2
     for i, (nt, na) in enumerate([(1 << 20, 51), (8, 128), (1024, 3072)]):
@@ -182,7 +184,7 @@ def test_plan_bytes_for_the_64_channel_three_product_workload(rlib):
     dev, host = C.c_size_t(0), C.c_size_t(0)
     assert rlib.lib().rawspec_amd_plan(C.byref(c), C.byref(dev), C.byref(host)) == 0
     MiB = 1 << 20
-    fixed = 256 * MiB + 2 * 1024 * MiB + 8 * MiB + 1 * MiB + 1536 * MiB
+    fixed = 256 * MiB + 2 * 1024 * MiB + 8 * MiB + 1 * MiB + 1536 * MiB + 8 * 1024 * MiB
     tw = (4096 + 1024) * 8          # master W_4096 table, pass A's fine table W_N^r, r < 1024
     assert fixed + tw <= dev.value <= fixed + tw + 64 * MiB, dev.value
     # pinned host: three dump buffers (blocks are allocated too: h_blkbufs NULL)
