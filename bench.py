@@ -22,7 +22,11 @@ keep the plain path's error bar).  The library runs this context's
 four-step and small products on two streams that overlap (rawspec_amd_set_
 lanes; --lanes off for one stream); the per-kernel times, and the roofline
 of the dominant kernel, come from a pass with HIP events around every
-kernel, in which the products run one after another.
+kernel, in which the products run one after another.  Beside the 8 TB/s
+vendor peak, roofline.measured_peaks gives what a plain 1 GiB device copy
+and fill reach on the same GPU, and roofline.traffic_GBps the dominant
+kernel's measured HBM bytes (committed rocprofv3 counter summary) over its
+launch time.
 
 Also measured, reported under "workloads": the other single-stream
 workloads (8 channels with 4 MiB buffers and Nfft 1024; the headline geometry
@@ -146,6 +150,34 @@ def measured_traffic(kernel, batch):
     if not k or rec.get("batch") != batch:
         return None
     return k.get("bytes_per_launch")
+
+
+def measured_peaks(torch, nbytes=1 << 30, reps=5):
+    """HBM rates this GPU reaches with plain device kernels, for comparison
+    with the 8 TB/s vendor peak: a device-to-device copy of nbytes (read +
+    write bytes counted) and a fill (write only), median of reps, timed with
+    events on torch's current stream (the one these kernels run on)."""
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").random_(0, 255)
+    dst = torch.empty_like(src)
+
+    def rate(fn, moved):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        ts = []
+        for _ in range(reps):
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return moved / sorted(ts)[len(ts) // 2] / 1e9
+
+    out = {"copy_GBps": round(rate(lambda: dst.copy_(src), 2 * nbytes), 1),
+           "fill_GBps": round(rate(lambda: dst.fill_(7), nbytes), 1),
+           "bytes": nbytes, "method": "median of 5 event-timed launches"}
+    del src, dst
+    torch.cuda.synchronize()
+    return out
 
 
 def submitter(sp, ring, batch):
@@ -428,6 +460,14 @@ def main(argv=None):
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(dom, a.batch if head_name != "plain" else 1),
                 "kernel": dom, "alg_bytes_per_launch": kb, "avg_launch_ms": dom_ms}
+    peaks = measured_peaks(torch)
+    roofline["measured_peaks"] = peaks
+    if roofline["traffic"]:
+        # the dominant kernel's measured HBM bytes per launch over its launch time
+        t_rate = roofline["traffic"] / (dom_ms * 1e-3) / 1e9
+        roofline["traffic_GBps"] = round(t_rate, 1)
+        roofline["traffic_frac_of_copy"] = round(t_rate / peaks["copy_GBps"], 3)
+    log(f"bench: device copy {peaks['copy_GBps']} GB/s, fill {peaks['fill_GBps']} GB/s")
     pipeline = {"alg_bytes_per_step": B_alg, "achieved_GBs": round(B_alg / (ms_per_step * 1e-3) / 1e9, 1),
                 "frac": round(B_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     log(f"bench: headline {head_name}; dominant kernel {dom} avg {dom_ms:.3f} ms")
