@@ -374,6 +374,51 @@ __device__ __forceinline__ void detect_add(float (&acc)[nq<PMODE>()], const floa
   }
 }
 
+// Four-step intermediate Y: written once by pass A and read once by the
+// vector pass B a whole launch later, when far more than the caches hold
+// has passed through them.  Bit 0 of RSK_NT_Y marks pass B's loads of Y
+// non-temporal (streaming: the lines do not displace cached data), bit 1
+// pass A's stores; both measured faster.  The matrix-core pass B reads Y
+// with plain loads: its waves share lines, which streaming loads would
+// fetch again.
+#ifndef RSK_NT_Y
+#define RSK_NT_Y 3
+#endif
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_y(float4 *p, float4 v) {
+  if constexpr (RSK_NT_Y & 2) {
+    const f32x4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<f32x4 *>(p));
+  } else {
+    *p = v;
+  }
+}
+__device__ __forceinline__ void store_y(float2 *p, float2 v) {
+  if constexpr (RSK_NT_Y & 2) {
+    const f32x2 t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<f32x2 *>(p));
+  } else {
+    *p = v;
+  }
+}
+__device__ __forceinline__ float4 load_y(const float4 *p) {
+  if constexpr (RSK_NT_Y & 1) {
+    const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4 *>(p));
+    return make_float4(t.x, t.y, t.z, t.w);
+  } else {
+    return *p;
+  }
+}
+__device__ __forceinline__ float2 load_y(const float2 *p) {
+  if constexpr (RSK_NT_Y & 1) {
+    const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2 *>(p));
+    return make_float2(t.x, t.y);
+  } else {
+    return *p;
+  }
+}
+
 // Bijective blockIdx -> logical tile map that hands each of the 8 XCDs a
 // contiguous range of tiles (hardware dispatches workgroups round-robin over
 // XCDs), so neighbouring tiles share one L2.

@@ -58,6 +58,7 @@ __device__ __forceinline__ unsigned y_off(unsigned j, unsigned n1, int m) {
     return ((k2 / TBY) * N1C + n1) * TBY + (k2 % TBY);
   }
 }
+
 // Pass A for N = N1 * N2 with N1 = R1 * N2 (R1 = 1 or 2).  The four-step
 // twiddle W_N^e, e = n1 * k2 < N, is split as e = h * N1 + r:
 // W_N^(h * N1) = W_N2^h is the sub-transform's own table (already in LDS)
@@ -150,9 +151,9 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
       const unsigned off = y_off<T, TBY, N1C>(j, n1, m);
       if constexpr (NP == 2) {
         const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
-        reinterpret_cast<float4 *>(dst)[off] = make_float4(y0.x, y0.y, y1.x, y1.y);
+        store_y(reinterpret_cast<float4 *>(dst) + off, make_float4(y0.x, y0.y, y1.x, y1.y));
       } else {
-        dst[off] = cmul(v[0][m], w);
+        store_y(dst + off, cmul(v[0][m], w));
       }
     }
     if (ln >= items) break;
@@ -207,11 +208,11 @@ void k_fs_b(FsArgs a) {
     for (int m = 0; m < E; ++m) {
       const unsigned long long off = ((unsigned long long)(j + m * T) * TB + sg) * NP;
       if constexpr (NP == 2) {
-        const float4 t = *reinterpret_cast<const float4 *>(y + off);
+        const float4 t = load_y(reinterpret_cast<const float4 *>(y + off));
         v[0][m] = make_float2(t.x, t.y);
         v[1][m] = make_float2(t.z, t.w);
       } else {
-        v[0][m] = y[off];
+        v[0][m] = load_y(y + off);
       }
     }
     team_fft<N1, E, NP, TwTab<N1>::TWN>(v, lds + sg * team_stride<N1, NP>(), j, twp);
