@@ -419,6 +419,22 @@ __device__ __forceinline__ float2 load_y(const float2 *p) {
   }
 }
 
+// The running sum of an integration in its internal layout: each block is
+// read and written once per pass-B launch by the one workgroup (or
+// wavefront) that owns it, so RSK_NT_ACC = 1 streams it past the caches
+// (measured 1% faster on the whole step).
+#ifndef RSK_NT_ACC
+#define RSK_NT_ACC 1
+#endif
+__device__ __forceinline__ float load_acc(const float *p) {
+  if constexpr (RSK_NT_ACC) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+__device__ __forceinline__ void store_acc(float *p, float v) {
+  if constexpr (RSK_NT_ACC) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Bijective blockIdx -> logical tile map that hands each of the 8 XCDs a
 // contiguous range of tiles (hardware dispatches workgroups round-robin over
 // XCDs), so neighbouring tiles share one L2.

@@ -240,7 +240,7 @@ void k_fs_b(FsArgs a) {
 #pragma unroll
     for (int m = 0; m < E; ++m)
 #pragma unroll
-      for (int q = 0; q < Q; ++q) prev[m][q] = ib[((unsigned long long)q * N1 + m * T) * TB];
+      for (int q = 0; q < Q; ++q) prev[m][q] = load_acc(ib + ((unsigned long long)q * N1 + m * T) * TB);
 #pragma unroll
     for (int m = 0; m < E; ++m)
 #pragma unroll
@@ -250,7 +250,7 @@ void k_fs_b(FsArgs a) {
 #pragma unroll
     for (int m = 0; m < E; ++m)
 #pragma unroll
-      for (int q = 0; q < Q; ++q) ib[((unsigned long long)q * N1 + m * T) * TB] = acc[m][q];
+      for (int q = 0; q < Q; ++q) store_acc(ib + ((unsigned long long)q * N1 + m * T) * TB, acc[m][q]);
   } else {
 #pragma unroll
     for (int m = 0; m < E; ++m) {

@@ -125,7 +125,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void
 #pragma unroll
       for (int r = 0; r < 16; ++r)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) prev[r][q] = ib[(q * 16 + r) * 64];
+        for (int q = 0; q < Q; ++q) prev[r][q] = load_acc(ib + (q * 16 + r) * 64);
 #pragma unroll
       for (int r = 0; r < 16; ++r)
 #pragma unroll
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void
 #pragma unroll
       for (int r = 0; r < 16; ++r)
 #pragma unroll
-        for (int q = 0; q < Q; ++q) ib[(q * 16 + r) * 64] = acc[r][q];
+        for (int q = 0; q < Q; ++q) store_acc(ib + (q * 16 + r) * 64, acc[r][q]);
       return;   // uniform across the workgroup: no wavefront reaches the barrier below
     }
   }
