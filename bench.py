@@ -18,7 +18,9 @@ timed: "plain", one buffer per rawspec_amd_start_processing_device() call
 vector units; "served", batches of 8 consecutive buffers per
 rawspec_amd_start_processing_device_batch() call; and "served_mfma", the
 same on the matrix-core engine.  `value` is the faster served path (both
-keep the plain path's error bar).  The library runs this context's
+keep the plain path's error bar).  paths.served_autotuned is the served
+path once more after rawspec_amd_autotune picked launch plans for batches
+of 8 (reported beside `value`, not in it).  The library runs this context's
 four-step and small products on two streams that overlap (rawspec_amd_set_
 lanes; --lanes off for one stream); the per-kernel times, and the roofline
 of the dominant kernel, come from a pass with HIP events around every
@@ -367,7 +369,7 @@ def main(argv=None):
     ap.add_argument("--ingest-blocks", type=int, default=32,
                     help="blocks of the GUPPI RAW file the rawspec program ingests (0: skip)")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
-    ap.add_argument("--legs", default="plain,served,served_mfma",
+    ap.add_argument("--legs", default="plain,served,served_mfma,served_autotuned",
                     help="paths to time (comma-separated subset of the default)")
     ap.add_argument("--lanes", choices=["auto", "on", "off"], default="auto",
                     help="two-lane execution of the headline context (auto: the library's "
@@ -491,6 +493,22 @@ def main(argv=None):
     sp.set_mfma(False)
     p50, p99 = lat_plain["p50"], lat_plain["p99"]
 
+    # the served path after rawspec_amd_autotune picked launch plans for
+    # batches of this size (last: the plans stay with the context)
+    tuned = None
+    if "served_autotuned" in want:
+        sp.reset_integration()
+        t1 = time.perf_counter()
+        rep = sp.autotune(ring[0].data_ptr(), a.batch)
+        tune_s = time.perf_counter() - t1
+        dtt, _ = timed(torch, sp, ring, a.steps, a.warmup, batch=a.batch)
+        tuned = {"GBps": round(a.steps * buf_bytes / dtt / 1e9, 2),
+                 "ms_per_step": round(dtt / a.steps * 1e3, 4),
+                 "vs_plain": round(a.steps * buf_bytes / dtt / 1e9 / legs["plain"]["GBps"], 3),
+                 "autotune_s": round(tune_s, 2),
+                 "plans": [ln.split(":")[0] for ln in rep.strip().splitlines()]}
+        log(f"bench: served after autotune ({tune_s:.2f} s): {tuned['GBps']} GB/s; {tuned['plans']}")
+
     extra = {}
     if a.pcie:
         # host-buffer API path: copy blocks from pinned host memory, dumps to h_pwrbuf
@@ -543,9 +561,10 @@ def main(argv=None):
                    "dumps": "device-resident",
                    "lanes": "four-step and small products on two streams" if lanes else "off",
                    "kernel_times": "HIP events from a pass with the products in sequence"},
-        "paths": {k: {"GBps": round(v["GBps"], 2), "ms_per_step": round(v["ms_per_step"], 4),
-                      "vs_plain": round(v["GBps"] / legs["plain"]["GBps"], 3)}
-                  for k, v in legs.items()},
+        "paths": dict({k: {"GBps": round(v["GBps"], 2), "ms_per_step": round(v["ms_per_step"], 4),
+                           "vs_plain": round(v["GBps"] / legs["plain"]["GBps"], 3)}
+                       for k, v in legs.items()},
+                      **({"served_autotuned": tuned} if tuned else {})),
         "roofline": roofline,
         "pipeline_roofline": pipeline,
         "kernels": kernels,
