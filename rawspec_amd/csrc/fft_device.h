@@ -422,7 +422,7 @@ __device__ __forceinline__ float2 load_y(const float2 *p) {
 // The running sum of an integration in its internal layout: each block is
 // read and written once per pass-B launch by the one workgroup (or
 // wavefront) that owns it, so RSK_NT_ACC = 1 streams it past the caches
-// (measured 1% faster on the whole step).
+// (measured: one-buffer calls 1% faster, batches unchanged).
 #ifndef RSK_NT_ACC
 #define RSK_NT_ACC 1
 #endif
