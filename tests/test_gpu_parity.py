@@ -44,6 +44,15 @@ def test_fourstep_lengths(nt):
          tone=(1, 20.0, 300.0, 1024.0))
 
 
+@pytest.mark.parametrize("nt", [1 << 21, 1 << 22, 1 << 23, 1 << 24])
+def test_fourstep_lengths_above_2_20(nt):
+    # N1 x N2 = 2048 x 1024 ... 4096 x 4096: pass-A teams of 128 and 256
+    # threads, pass-B tiles of 2 and 1 columns, one spectrum per buffer,
+    # integrated over two buffers (held first-pass output)
+    _run(Nc=2, Ntpb=nt, Nts=[nt], Nas=[2], Npolout=[-4], nbuf=2,
+         tone=(1, 20.0, 300.0, 1024.0))
+
+
 def test_three_products_pols4_inverse_conj():
     # rawspectest-like product set at reduced size: 2^17, 8, 1024
     _run(Nc=2, Ntpb=65536, Nts=[131072, 8, 1024], Nas=[3, 128, 256],
