@@ -139,17 +139,18 @@ def alg_bytes(sp, buf_bytes):
     return buf_bytes + out, per_prod
 
 
-def measured_traffic(kernel, batch):
+def measured_traffic(kernel, batch, engine):
     """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by
-    the counter's reading for a 1 GiB copy), or None when none matches."""
-    path = os.path.join(ROOT, "profiles", "stokes_64ch_int8_3products_hbm_traffic.json")
+    rocprofv3 --pmc summary of the same leg (FETCH_SIZE and WRITE_SIZE runs,
+    each scaled by the counter's reading for a 1 GiB copy), or None when no
+    summary of that engine and batch size holds the kernel."""
+    name = "stokes_64ch_int8_3products_hbm_traffic" + ("_mfma" if engine == "mfma" else "") + ".json"
     try:
-        rec = json.load(open(path))
+        rec = json.load(open(os.path.join(ROOT, "profiles", name)))
     except (OSError, ValueError):
         return None
     k = rec.get("kernels", {}).get(kernel)
-    if not k or rec.get("batch") != batch:
+    if not k or rec.get("batch") != batch or rec.get("engine") != engine:
         return None
     return k.get("bytes_per_launch")
 
@@ -405,8 +406,11 @@ def main(argv=None):
         f"lanes {'on' if lanes else 'off'}")
 
     def measure(batch, mfma):
-        """Throughput of one leg, then its per-kernel HIP-event timing."""
-        sp.set_mfma(mfma)          # takes effect as each product starts an integration
+        """Throughput of one leg, then its per-kernel HIP-event timing.  The
+        engine applies from each product's next integration start, so every
+        integration is restarted first: the leg times only its own engine."""
+        assert sp.set_mfma(mfma) == 0
+        assert sp.reset_integration() == 0
         dt, go = timed(torch, sp, ring, a.steps, a.warmup, batch=batch)
         sp.profile(True)
         nprof = max(batch, (max(51, min(a.steps, 152)) // batch) * batch)
@@ -415,6 +419,12 @@ def main(argv=None):
         sp.profile(False)
         kern = {k: {"launches": n, "avg_ms": ms / n, "total_ms": ms, "ms_per_step": ms / nprof}
                 for k, (n, ms) in prof.items()}
+        if mfma:
+            # the matrix-core leg runs every product that has a matrix-core
+            # kernel on it: no vector pass B, no vector 1024-point kernel
+            vec = [k for k in kern if k.split(".")[1] == "k_fs_b" or
+                   (k.split(".")[1] == "k_small" and k.endswith(".N1024"))]
+            assert not vec, f"matrix-core leg launched vector kernels {vec}"
         gbs = a.steps * buf_bytes / dt / 1e9
         log(f"bench: batch {batch} mfma {int(mfma)}: {a.steps} steps in {dt:.3f} s -> {gbs:.1f} GB/s "
             f"({dt / a.steps * 1e3:.3f} ms/step); " +
@@ -434,7 +444,7 @@ def main(argv=None):
     legs = {k: measure(*spec[k]) for k in spec if k in want}
     if "plain" not in legs:
         legs["plain"] = legs[want[0]]
-    sp.set_mfma(False)
+    assert sp.set_mfma(False) == 0 and sp.reset_integration() == 0
     head_name = max([k for k in ("served", "served_mfma") if k in legs] or ["plain"],
                     key=lambda k: legs[k]["GBps"])
     head = legs[head_name]
@@ -460,7 +470,8 @@ def main(argv=None):
     achieved = kb / (dom_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(dom, a.batch if head_name != "plain" else 1),
+                "traffic": measured_traffic(dom, a.batch if head_name != "plain" else 1,
+                                            "mfma" if head_name == "served_mfma" else "vector"),
                 "kernel": dom, "alg_bytes_per_launch": kb, "avg_launch_ms": dom_ms}
     peaks = measured_peaks(torch)
     roofline["measured_peaks"] = peaks
@@ -488,9 +499,9 @@ def main(argv=None):
                 "p99": round(lat[min(len(lat) - 1, int(round(0.99 * (len(lat) - 1))))], 3)}
 
     lat_plain = latency(submitter(sp, ring, 1), 1)
-    sp.set_mfma(head_name == "served_mfma")
+    assert sp.set_mfma(head_name == "served_mfma") == 0 and sp.reset_integration() == 0
     lat_served = latency(submitter(sp, ring, a.batch), a.batch)
-    sp.set_mfma(False)
+    assert sp.set_mfma(False) == 0 and sp.reset_integration() == 0
     p50, p99 = lat_plain["p50"], lat_plain["p99"]
 
     # the served path after rawspec_amd_autotune picked launch plans for
@@ -550,7 +561,8 @@ def main(argv=None):
     res = {
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "split-fp16 (MFMA, fp32 accumulate)" if head_name == "served_mfma" else "fp32",
         "data": "synthetic",
         "config": {"workload": "stokes_64ch_int8_3products: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
                                "blocks of 524288 samples, Nb=2 (256 MiB/step), "

@@ -111,8 +111,11 @@ int rawspec_amd_get_held_buffers(rawspec_context *ctx);
  * how work is split over workgroups and launches, so results stay within
  * the same error bars.  The timing runs make no dumps and fire no callbacks;
  * they overwrite the integration buffers, so every integration restarts
- * with the next buffer, as after rawspec_reset_integration().  May grow the
- * context's scratch buffers.  Writes one line per product ("p<i> N<Nt>
+ * with the next buffer, as after rawspec_reset_integration().  Candidates
+ * may grow the context's scratch buffers while they are timed; on return
+ * the scratch buffers are sized for the chosen plans (more than
+ * rawspec_amd_plan() reported only where a chosen plan needs more than the
+ * default one).  Writes one line per product ("p<i> N<Nt>
  * <plan>: <best> ms (default <ms> ms, <n> candidates)", milliseconds per
  * buffer of the whole sequence) into report (NUL-terminated; NULL with len 0
  * for none).  Returns 0 on success. */

@@ -437,6 +437,17 @@ static int rs_dump(rawspec_context *ctx, rs_gpu_ctx *g, unsigned int i)
   return 0;
 }
 
+/* A start call that fails after forking the lanes joins them back into the
+ * context stream (so rawspec_wait_for_completion() also waits for the
+ * kernels already queued there, which read the input buffer) and takes back
+ * the buffers it counted (count: how many). */
+static int rs_abandon(rs_gpu_ctx *g, unsigned int count)
+{
+  (void)rsd_lane_end(g->dev, -1);
+  g->inbuf_count -= count;
+  return 1;
+}
+
 /* The products' kernels run on the device layer's lanes (four-step and small
  * products side by side); every dump waits for its product's lane and runs,
  * in product order, on the context stream, which waits for both lanes before
@@ -447,19 +458,20 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
   int last[MAX_OUTPUTS];
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
-  g->inbuf_count++;
   if (rsd_lanes_begin(g->dev)) return 1;
+  g->inbuf_count++;
   for (i = 0; i < ctx->No; i++) {
     int first = ((g->inbuf_count - 1) % g->Nis[i]) == 0;
     last[i] = (g->inbuf_count % g->Nis[i]) == 0;
     if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first, last[i])) {
       fprintf(stderr, "librawspec: processing output product %u failed\n", i);
       fflush(stderr);
-      return 1;
+      return rs_abandon(g, 1);
     }
   }
   for (i = 0; i < ctx->No; i++)
-    if (last[i] && !g->tuning && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i))) return 1;
+    if (last[i] && !g->tuning && (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i)))
+      return rs_abandon(g, 1);
   return rsd_lane_end(g->dev, -1);
 }
 
@@ -484,13 +496,13 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
       if (rsd_run_product_batch(g->dev, (int)i, d_in + k, len, fft_dir <= 0, pos == 0, last)) {
         fprintf(stderr, "librawspec: processing output product %u failed\n", i);
         fflush(stderr);
-        return 1;
+        return rs_abandon(g, 0);
       }
       /* the dump reads the integration buffer the lane's next piece rewrites */
       if (last && !g->tuning &&
           (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i) ||
            (k + len < nbuf && rsd_lane_resume(g->dev, (int)i))))
-        return 1;
+        return rs_abandon(g, 0);
       k += len;
     }
   }
@@ -726,6 +738,9 @@ int rawspec_amd_autotune(rawspec_context *ctx, const void *d_in, unsigned int pe
   g->tuning = 0;
   /* the timing runs overwrote the integration buffers */
   g->inbuf_count = 0;
+  /* candidates may have grown the scratch buffers beyond the chosen plans'
+   * needs (and beyond what rawspec_amd_plan reports) */
+  if (rsd_tune_fit(g->dev)) rc = 1;
   return rc;
 }
 

@@ -130,6 +130,9 @@ unsigned rsd_get_hold(rsd_ctx *d);
 int rsd_tune_count(rsd_ctx *d, int i);
 int rsd_tune_apply(rsd_ctx *d, int i, int k);
 int rsd_tune_describe(rsd_ctx *d, int i, char *buf, size_t len);
+/* Shrinks the scratch buffers to what the products' current plans need
+ * (after tuning, which grows them for larger candidates). */
+int rsd_tune_fit(rsd_ctx *d);
 
 #ifdef __cplusplus
 }

@@ -897,6 +897,36 @@ int rsd_tune_apply(rsd_ctx *d, int i, int k) {
   return fit_buffers(d, pp, work);
 }
 
+int rsd_tune_fit(rsd_ctx *d) {
+  // what the products' current plans need, at most what is allocated
+  size_t part = 0, work = 0;
+  for (unsigned i = 0; i < d->cfg.No; ++i) {
+    ProdPlan &pp = d->pp[i];
+    const size_t w = plan_product(d, pp);
+    if (w > work) work = w;
+    const size_t pn = part_need(d, pp);
+    if (pn > part) part = pn;
+  }
+  if (part >= d->part_len && work >= d->work_len) return 0;
+  RSD_CHECK(hipSetDevice(d->dev));
+  RSD_CHECK(hipStreamSynchronize(d->stream));
+  if (part < d->part_len) {
+    if (d->d_part) RSD_CHECK(hipFree(d->d_part));
+    d->d_part = nullptr;
+    d->part_len = 0;
+    if (alloc_zero((void **)&d->d_part, part * sizeof(float))) return 1;
+    d->part_len = part;
+  }
+  if (work < d->work_len) {
+    if (d->d_work) RSD_CHECK(hipFree(d->d_work));
+    d->d_work = nullptr;
+    d->work_len = 0;
+    if (alloc_zero((void **)&d->d_work, work * sizeof(float2))) return 1;
+    d->work_len = work;
+  }
+  return 0;
+}
+
 int rsd_tune_describe(rsd_ctx *d, int i, char *buf, size_t len) {
   const ProdPlan &pp = d->pp[i];
   if (pp.small)

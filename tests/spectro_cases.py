@@ -16,6 +16,12 @@ import numpy as np
 import oracle as O
 
 PLAIN_TOL = 1e-5      # plain FP32 path vs FP64 oracle (max over all bins)
+# Four-step products (N >= 8192) are held to max(PLAIN_TOL, k x the error of
+# an FP32 FFT on the same input) -- a strong tone puts FP32 rounding spurs
+# above 1e-5 in any single-precision FFT of that length -- and never above
+# this fixed ceiling, so an error that grows with the input's FP32 floor
+# still fails.
+FOURSTEP_CEIL = 5e-5
 
 
 def gen_blocks(Nc, Ntpb, Np, Nbps, nblocks, seed=1, sigma=8.0, tone=None, first_blk=0):
@@ -169,10 +175,23 @@ def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
 MFMA_FLOOR_MULT = 4.0
 
 
-def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0):
+def bin_rel_err(gpu: np.ndarray, ref: np.ndarray) -> float:
+    """max |gpu - ref| / ref over the bins of a non-negative component (the
+    per-bin metric without the channel-mean floor), for reporting."""
+    g = np.asarray(gpu, dtype=np.float64)
+    r = np.asarray(ref, dtype=np.float64)
+    return float((np.abs(g - r) / np.maximum(r, 1e-30)).max())
+
+
+def nonneg_components(mode: int):
+    """Components that are powers (never negative): P; XX, YY; I."""
+    return [0] if mode in (1, -4) else [0, 1]
+
+
+def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0, ceil=FOURSTEP_CEIL):
     """Asserts every dump is within tol (four-step products: within
-    max(tol, floor_mult x the FP32 reference's error)); returns max error per
-    product."""
+    min(ceil, max(tol, floor_mult x the FP32 reference's error))); returns
+    max error per product."""
     worst = {}
     for (n, i, g, r, gi, ri, f32) in results:
         q = abs(cfg.Npolout[i])
@@ -182,7 +201,7 @@ def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0):
         t = tol
         if f32 is not None:
             floor = float(rel_err(f32, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i]).max())
-            t = max(tol, floor_mult * floor)
+            t = min(ceil, max(tol, floor_mult * floor))
             print(f"  product {i}: gpu {e.max():.2e}  fp32-reference floor {floor:.2e}  bound {t:.2e}")
         assert e.max() <= t, f"product {i} buffer {n}: max rel err {e.max():.3e} > {t}"
         if ics:

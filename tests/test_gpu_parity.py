@@ -90,24 +90,38 @@ def test_sixteen_antennas_int16_four_products_ics():
                                                       (1, 2), (1, 3)]
 
 
-@pytest.mark.parametrize("mfma", [False, True])
-def test_stokes_three_products_full_integration_lengths(mfma):
-    """The 64-channel Stokes product set at its full integration lengths on
+@pytest.mark.parametrize("mode,mfma", [(1, False), (4, False), (-4, False), (-4, True)],
+                         ids=["power", "crosspol", "stokes", "stokes-mfma"])
+def test_three_products_full_integration_lengths(mode, mfma):
+    """The 64-channel three-product set at its full integration lengths on
     one of its channels (channels are independent; the tone's): blocks of
     524288 samples, two per buffer, Nfft {2^20, 8, 1024} x Nint {51, 128,
     3072} over 51 buffers, so the 2^20 product integrates across 51 buffers
     (first-pass output held across the one-buffer calls) and the 1024-point
-    product over 3072 spectra across 3 buffers; on the matrix cores, 9
+    product over 3072 spectra across 3 buffers; pols 1 (total power), 4
+    (XX, YY, Re XY*, Im XY*) and -4 (I, Q, U, V); on the matrix cores, 9
     buffers with Nint {9, 128, 3072}.  FP32 sums in blocks (registers, a
     fixed-order tree across slots, one add per buffer or held group) keep
     the 3072-spectrum integration within the plain bar without compensated
-    summation."""
+    summation.  Beside the bar's metric, the per-bin |dP|/P of the power
+    components (no channel-mean floor) is printed."""
+    from tests.spectro_cases import bin_rel_err, nonneg_components
     nbuf = 9 if mfma else 51
+    print(f"case: pols {mode} mfma {mfma} nbuf {nbuf}", flush=True)
     res, cfg, ev = run_case(Nc=1, Ntpb=524288, Nts=[1 << 20, 8, 1024], Nas=[nbuf, 128, 3072],
-                            Npolout=[-4, -4, -4], nbuf=nbuf, tone=(0, 20.0, 300.0, 1024.0),
+                            Npolout=[mode] * 3, nbuf=nbuf, tone=(0, 20.0, 300.0, 1024.0),
                             mfma=mfma)
     worst = check_results(res, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT if mfma else 2.0)
     print(f"  dumps={len(res)} worst={ {k: f'{v:.2e}' for k, v in worst.items()} }", flush=True)
+    perbin = {}
+    for (n, i, g, r, *_rest) in res:
+        q = abs(cfg.Npolout[i])
+        g4 = g.reshape(cfg.Nds[i], q, cfg.Nc, cfg.Nts[i])
+        r4 = r.reshape(cfg.Nds[i], q, cfg.Nc, cfg.Nts[i])
+        for k in nonneg_components(mode):
+            perbin[i] = max(perbin.get(i, 0.0), bin_rel_err(g4[:, k], r4[:, k]))
+    print(f"  per-bin |dP|/P of the power components: "
+          f"{ {k: f'{v:.2e}' for k, v in perbin.items()} }", flush=True)
     assert [sum(1 for r in res if r[1] == i) for i in range(3)] == [1, nbuf, nbuf // 3]
 
 
