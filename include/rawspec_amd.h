@@ -41,6 +41,25 @@ size_t rawspec_amd_device_input_size(rawspec_context *ctx);
 int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in,
                                         int fft_dir);
 
+/* As rawspec_amd_start_processing_device(), but the call allocates nothing
+ * (no hipMalloc, hipFree or allocating library routine), for callers that
+ * cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation.  No counterpart in the reference.  The
+ * kernels keep their partial sums and the four-step intermediate in a static
+ * device array of 256 MiB that the library holds on each GPU it runs on until
+ * the process ends (the context's own buffers, allocated by
+ * rawspec_initialize(), still hold input, integrations and dumps).  Four-step
+ * products transform as many channels per launch as that scratch holds, and
+ * an integration that starts in this call does not hold first-pass output
+ * unless an earlier call allocated the store.  A configuration whose partial
+ * sums, or one channel's intermediate, exceed the scratch is refused with a
+ * message (returns 1, nothing enqueued).  Every call on a GPU shares that
+ * GPU's scratch: calls that overlap on one GPU (contexts on different
+ * streams) race; calls on one context are ordered by its stream.  Results
+ * equal rawspec_amd_start_processing_device()'s with held output off. */
+int rawspec_amd_start_processing_device_noalloc(rawspec_context *ctx, const void *d_in,
+                                                int fft_dir);
+
 /* As nbuf calls of rawspec_amd_start_processing_device(), one per buffer of
  * d_in[0 .. nbuf-1] in order: the same integrations, the same dumps, each
  * product's PRE/POST callbacks in the same order.  The device work is

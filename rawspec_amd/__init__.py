@@ -199,6 +199,7 @@ def lib():
             ("rawspec_amd_device_input", [P], C.c_void_p),
             ("rawspec_amd_device_input_size", [P], C.c_size_t),
             ("rawspec_amd_start_processing_device", [P, C.c_void_p, C.c_int], C.c_int),
+            ("rawspec_amd_start_processing_device_noalloc", [P, C.c_void_p, C.c_int], C.c_int),
             ("rawspec_amd_start_processing_device_batch",
              [P, C.POINTER(C.c_void_p), C.c_uint, C.c_int], C.c_int),
             ("rawspec_amd_set_device_output", [P, C.c_int], C.c_int),
@@ -304,6 +305,10 @@ class Spectrometer:
     def start_processing_device(self, dev_ptr: int, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
         return lib().rawspec_amd_start_processing_device(C.byref(self.ctx), C.c_void_p(dev_ptr),
                                                          fft_dir)
+
+    def start_processing_device_noalloc(self, dev_ptr: int, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
+        return lib().rawspec_amd_start_processing_device_noalloc(C.byref(self.ctx),
+                                                                 C.c_void_p(dev_ptr), fft_dir)
 
     def start_processing_device_batch(self, dev_ptrs, fft_dir=RAWSPEC_FORWARD_FFT) -> int:
         arr = (C.c_void_p * len(dev_ptrs))(*dev_ptrs)

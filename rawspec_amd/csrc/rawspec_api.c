@@ -579,6 +579,23 @@ int rawspec_amd_start_processing_device(rawspec_context *ctx, const void *d_in, 
   return rs_start(ctx, d_in, fft_dir);
 }
 
+int rawspec_amd_start_processing_device_noalloc(rawspec_context *ctx, const void *d_in,
+                                                int fft_dir)
+{
+  int rc;
+  rs_gpu_ctx *g;
+  if (!d_in || ((size_t)d_in & 15) != 0) {
+    rs_err("rawspec_amd_start_processing_device_noalloc: input must be a 16-byte aligned device pointer");
+    return 1;
+  }
+  g = rs_get(ctx);
+  if (!g) return 1;
+  if (rsd_noalloc_begin(g->dev)) return 1;
+  rc = rs_start(ctx, d_in, fft_dir);
+  rsd_noalloc_end(g->dev);
+  return rc;
+}
+
 int rawspec_amd_start_processing_device_batch(rawspec_context *ctx, const void *const *d_in,
                                               unsigned int nbuf, int fft_dir)
 {

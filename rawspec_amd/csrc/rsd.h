@@ -130,6 +130,14 @@ unsigned rsd_get_hold(rsd_ctx *d);
 int rsd_tune_count(rsd_ctx *d, int i);
 int rsd_tune_apply(rsd_ctx *d, int i, int k);
 int rsd_tune_describe(rsd_ctx *d, int i, char *buf, size_t len);
+/* Allocation-free processing: between rsd_noalloc_begin and rsd_noalloc_end
+ * the context's kernels take their partial sums and four-step intermediate
+ * from the device's static scratch (a fixed size, shared by every call on
+ * that GPU), never allocate held first-pass output and record no timing
+ * events.  begin fails, with a message, when one channel's intermediate or
+ * the partial sums do not fit; end restores the context's own buffers. */
+int rsd_noalloc_begin(rsd_ctx *d);
+void rsd_noalloc_end(rsd_ctx *d);
 /* Shrinks the scratch buffers to what the products' current plans need
  * (after tuning, which grows them for larger candidates). */
 int rsd_tune_fit(rsd_ctx *d);

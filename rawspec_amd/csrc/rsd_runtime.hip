@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -154,6 +155,15 @@ struct rsd_ctx {
   ProdPlan pp[RSD_MAX_PRODUCTS];
   // kernel timing (rsd_profile)
   int mfma;                            // matrix-core engine requested (rsd_set_mfma)
+  // allocation-free call in progress (rsd_noalloc_begin .. rsd_noalloc_end):
+  // d_part and d_work point into the device's static scratch, and the
+  // context's own buffers and four-step budgets are parked here
+  bool noalloc;
+  float *sv_part;
+  size_t sv_part_len;
+  float2 *sv_work;
+  size_t sv_work_len;
+  size_t sv_budget[RSD_MAX_PRODUCTS];
   bool prof;
   struct Rec { std::string name; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -189,7 +199,7 @@ struct ProfScope {
       : ProfScope(dd, product, kname, n, dd->stream) {}
   ProfScope(rsd_ctx *dd, int product, const char *kname, unsigned n, hipStream_t st)
       : d(dd), idx(-1), s(st) {
-    if (!d->prof) return;
+    if (!d->prof || d->noalloc) return;
     hipEvent_t a = ev_take(d), b = ev_take(d);
     if (!a || !b) return;
     char nm[64];
@@ -258,6 +268,48 @@ static size_t part_need(const rsd_ctx *d, const ProdPlan &pp) {
     if (m > n) n = m;
   }
   return n;
+}
+
+// ---- static scratch of the allocation-free entry point ---------------------
+// rawspec_amd_start_processing_device_noalloc runs the same kernels with the
+// small products' partial sums and the four-step intermediate in this array
+// instead of the context's own scratch buffers, so the call allocates
+// nothing.  Each GPU holds its own copy (the code object's) from module load
+// until the process ends; its device address is looked up once per device at
+// context creation (hipGetSymbolAddress) and only that address is handed to
+// kernels.  Nothing in it must start at zero: every kernel of a call reads
+// only what an earlier kernel of the same call wrote.  All calls on one GPU
+// share it, so allocation-free calls that overlap on one GPU (different
+// contexts' streams) race.
+#ifndef RSD_SCRATCH_MB
+#define RSD_SCRATCH_MB 256
+#endif
+constexpr size_t kScratchBytes = (size_t)RSD_SCRATCH_MB << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kScratchDevices = 64;
+static void *g_scratch_ptr[kScratchDevices];
+static std::mutex g_scratch_mu;
+
+// Device address of the scratch on device dev (the current device).
+static int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    fprintf(stderr, "librawspec: device %d is beyond the %d devices the scratch table covers\n", dev,
+            kScratchDevices);
+    fflush(stderr);
+    return 1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    fprintf(stderr, "librawspec: hipGetSymbolAddress(scratch) failed on device %d: %s\n", dev,
+            hipGetErrorString(e));
+    fflush(stderr);
+    return 1;
+  }
+  g_scratch_ptr[dev] = p;
+  return 0;
 }
 
 static int alloc_zero(void **p, size_t bytes) {
@@ -442,6 +494,7 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
   if (plan_all(cfg, d)) return 1;
   if (cfg->ics) d->weights.assign(cfg->weights, cfg->weights + cfg->Nant);
   RSD_CHECK(hipSetDevice(d->dev));
+  if (scratch_setup(d->dev)) return 1;
   {
     int cus = 0;
     RSD_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->dev));
@@ -506,6 +559,7 @@ static rsd_ctx *new_ctx() {
   d->d_w = nullptr;
   d->prof = false;
   d->mfma = 0;
+  d->noalloc = false;
   return d;
 }
 
@@ -713,6 +767,7 @@ static int hold_ready(rsd_ctx *d, ProdPlan &pp) {
   const unsigned k = pp.hold_k < d->hold_max ? pp.hold_k : d->hold_max;
   pp.held = 0;
   if (k < 2 || pp.hold_failed) return 0;
+  if (!pp.d_held && d->noalloc) return 0;   // holds only where the store exists
   if (!pp.d_held) {
     const size_t bytes = (size_t)pp.hold_k * d->cfg.Nc * pp.p.Nss * pp.p.Nt * d->cfg.Np *
                          sizeof(float2);
@@ -895,6 +950,69 @@ int rsd_tune_apply(rsd_ctx *d, int i, int k) {
   RSD_CHECK(hipSetDevice(d->dev));
   RSD_CHECK(hipStreamSynchronize(d->stream));
   return fit_buffers(d, pp, work);
+}
+
+int rsd_noalloc_begin(rsd_ctx *d) {
+  if (d->noalloc) return 1;
+  void *base = (d->dev >= 0 && d->dev < kScratchDevices) ? g_scratch_ptr[d->dev] : nullptr;
+  if (!base) {
+    fprintf(stderr, "librawspec: no allocation-free scratch on device %d\n", d->dev);
+    fflush(stderr);
+    return 1;
+  }
+  // the small products' partial sums first, then the four-step intermediate
+  size_t part = 0;
+  for (unsigned i = 0; i < d->cfg.No; ++i) {
+    const size_t pn = part_need(d, d->pp[i]);
+    if (pn > part) part = pn;
+  }
+  const size_t part_bytes = (part * sizeof(float) + 255) / 256 * 256;
+  if (part_bytes > kScratchBytes) {
+    fprintf(stderr,
+            "librawspec: allocation-free processing needs %zu MiB of partial sums; the scratch "
+            "holds %zu MiB\n", (part_bytes + (1u << 20) - 1) >> 20, kScratchBytes >> 20);
+    fflush(stderr);
+    return 1;
+  }
+  const size_t work_bytes = kScratchBytes - part_bytes;
+  for (unsigned i = 0; i < d->cfg.No; ++i) {
+    const ProdPlan &pp = d->pp[i];
+    if (pp.small) continue;
+    const size_t per_chan = (size_t)pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
+    if (per_chan > work_bytes) {
+      fprintf(stderr,
+              "librawspec: allocation-free processing needs %zu MiB of intermediate for one "
+              "channel of product %u; the scratch holds %zu MiB beside %zu MiB of partial sums\n",
+              per_chan >> 20, i, kScratchBytes >> 20, part_bytes >> 20);
+      fflush(stderr);
+      return 1;
+    }
+  }
+  d->sv_part = d->d_part;
+  d->sv_part_len = d->part_len;
+  d->sv_work = d->d_work;
+  d->sv_work_len = d->work_len;
+  d->d_part = reinterpret_cast<float *>(base);
+  d->part_len = part;
+  d->d_work = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(base) + part_bytes);
+  d->work_len = work_bytes / sizeof(float2);
+  for (unsigned i = 0; i < d->cfg.No; ++i) {
+    ProdPlan &pp = d->pp[i];
+    d->sv_budget[i] = pp.fs_budget;
+    if (!pp.small && pp.fs_budget > work_bytes) pp.fs_budget = work_bytes;
+  }
+  d->noalloc = true;
+  return 0;
+}
+
+void rsd_noalloc_end(rsd_ctx *d) {
+  if (!d->noalloc) return;
+  d->d_part = d->sv_part;
+  d->part_len = d->sv_part_len;
+  d->d_work = d->sv_work;
+  d->work_len = d->sv_work_len;
+  for (unsigned i = 0; i < d->cfg.No; ++i) d->pp[i].fs_budget = d->sv_budget[i];
+  d->noalloc = false;
 }
 
 int rsd_tune_fit(rsd_ctx *d) {
