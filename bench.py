@@ -328,7 +328,14 @@ def file_ingest(torch, W, nblocks):
         out["h2d_pinned_GBps"] = round(8 * n / (time.perf_counter() - t1) / 1e9, 2)
         out["pipelined_vs_reference_order"] = round(out["pipelined"]["GBps"] /
                                                     out["reference_order"]["GBps"], 2)
+        # Release the pinned block here, while the runtime is fully up, rather
+        # than leave it in torch's host-allocator cache to be freed by a static
+        # destructor during process exit, after other exit handlers (such as a
+        # profiler's, which wraps the runtime's memory calls) have run.
+        torch.cuda.synchronize()
         del h, dv
+        torch._C._host_emptyCache()
+        torch.cuda.empty_cache()
     finally:
         shutil.rmtree(d, ignore_errors=True)
     return out

@@ -88,12 +88,17 @@ float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i);
 void *rawspec_amd_stream(rawspec_context *ctx);
 
 /* Transform engine of the context.  enable = 0: FP32 on the vector units
- * (the default); enable != 0: the matrix cores (FP16 MFMA with hi/lo split
- * operands and FP32 accumulation, whose error stays at the FP32 level)
- * wherever a matrix-core kernel exists for the product's shape, the vector
- * units elsewhere.  A product switches engine at the start of its next
- * integration, so a partial integration is never mixed.  The getter returns
- * the requested setting, -1 before initialisation. */
+ * (the default).  enable = 1: the matrix cores (FP16 MFMA with FP32
+ * accumulation) wherever a matrix-core kernel exists for the product's shape,
+ * the vector units elsewhere, held to the engine's 1e-3 relative error bar:
+ * int8 products that integrate at least 32 spectra per dump use unsplit FP16
+ * operands (about 2^-12 rounding per transform, random from spectrum to
+ * spectrum: the error of an integration falls with its length, to a floor
+ * near 1.5e-4), all others hi/lo split operands.  enable = 2: the matrix
+ * cores with split operands everywhere (error at the FP32 level).  Other
+ * values are refused.  A product longer than 4096 points switches engine at
+ * the start of its next integration, so a partial integration is never
+ * mixed.  The getter returns the setting, -1 before initialisation. */
 int rawspec_amd_set_mfma(rawspec_context *ctx, int enable);
 int rawspec_amd_get_mfma(rawspec_context *ctx);
 

@@ -330,8 +330,11 @@ class Spectrometer:
     def stream(self) -> int:
         return lib().rawspec_amd_stream(C.byref(self.ctx)) or 0
 
-    def set_mfma(self, enable: bool) -> int:
-        return lib().rawspec_amd_set_mfma(C.byref(self.ctx), 1 if enable else 0)
+    def set_mfma(self, level) -> int:
+        """0 / False: vector units; 1 / True: matrix cores at the 1e-3 bar
+        (unsplit operands for long int8 integrations); 2: matrix cores with
+        split operands everywhere."""
+        return lib().rawspec_amd_set_mfma(C.byref(self.ctx), int(level))
 
     def get_mfma(self) -> int:
         return lib().rawspec_amd_get_mfma(C.byref(self.ctx))

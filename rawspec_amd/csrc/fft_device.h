@@ -313,7 +313,16 @@ struct Sample<8, 2> {
     o[0] = make_float2((float)a, isign * (float)b);
     o[1] = make_float2((float)c, isign * (float)d);
   }
+  __device__ __forceinline__ static void cvt_offset(word w, float2 (&o)[2], float isign);
 };
+// int8 samples offset by 128 (bytes XOR 0x80, so each converts with one
+// unsigned-byte conversion), clamped at 1 (the SNORM clamp of -128); the
+// offset 128 (1 + i isign) is removed from bin 0 after the transform.  Exact
+// for single-stage transforms (N <= 16): on integers every butterfly
+// difference cancels the offset exactly, so only bin 0 carries it.
+__device__ __forceinline__ float u8f(unsigned int t, int byte) {
+  return fmaxf((float)((t >> (8 * byte)) & 0xffu), 1.0f);
+}
 template <>
 struct Sample<8, 1> {
   using word = uint16_t;
@@ -322,7 +331,16 @@ struct Sample<8, 1> {
     a = max(a, -127); b = max(b, -127);
     o[0] = make_float2((float)a, isign * (float)b);
   }
+  __device__ __forceinline__ static void cvt_offset(word w, float2 (&o)[1], float isign) {
+    const unsigned int t = (unsigned int)w ^ 0x8080u;
+    o[0] = make_float2(u8f(t, 0), isign * u8f(t, 1));
+  }
 };
+__device__ __forceinline__ void Sample<8, 2>::cvt_offset(word w, float2 (&o)[2], float isign) {
+  const unsigned int t = w ^ 0x80808080u;
+  o[0] = make_float2(u8f(t, 0), isign * u8f(t, 1));
+  o[1] = make_float2(u8f(t, 2), isign * u8f(t, 3));
+}
 template <>
 struct Sample<16, 2> {
   using word = uint2;
@@ -351,26 +369,44 @@ struct Sample<16, 1> {
 template <int PMODE>
 constexpr int nq() { return PMODE == 1 ? 1 : 4; }
 
-template <int PMODE, int NP>
-__device__ __forceinline__ void detect_add(float (&acc)[nq<PMODE>()], const float2 (&x)[NP],
-                                           float gam) {
-  const float p0 = x[0].x * x[0].x + x[0].y * x[0].y;
+// Detection into raw sums, fused multiply-adds only: PMODE 1 accumulates the
+// total power; 4 and -4 accumulate XX, YY, Re X0 X1* and Im X0 X1* (no sign
+// applied), and detect_finish turns the sums into the product's components
+// (-4: I = XX + YY, Q = XX - YY; both: the sign gam on the imaginary cross
+// term).  The finishing step is linear, so it may be applied to any sum of
+// raw sums.
+template <int PMODE>
+__device__ __forceinline__ void detect_raw(float (&acc)[nq<PMODE>()], float xr0, float xi0,
+                                           float xr1, float xi1) {
   if constexpr (PMODE == 1) {
-    if constexpr (NP == 2) acc[0] += p0 + (x[1].x * x[1].x + x[1].y * x[1].y);
-    else acc[0] += p0;
+    acc[0] = fmaf(xi1, xi1, fmaf(xr1, xr1, fmaf(xi0, xi0, fmaf(xr0, xr0, acc[0]))));
   } else {
-    const float p1 = x[1].x * x[1].x + x[1].y * x[1].y;
-    const float re = x[0].x * x[1].x + x[0].y * x[1].y;
-    const float im = gam * (x[0].y * x[1].x - x[0].x * x[1].y);
-    if constexpr (PMODE == 4) {
-      acc[0] += p0;
-      acc[1] += p1;
-    } else {
-      acc[0] += p0 + p1;
-      acc[1] += p0 - p1;
+    acc[0] = fmaf(xi0, xi0, fmaf(xr0, xr0, acc[0]));
+    acc[1] = fmaf(xi1, xi1, fmaf(xr1, xr1, acc[1]));
+    acc[2] = fmaf(xi0, xi1, fmaf(xr0, xr1, acc[2]));
+    acc[3] = fmaf(-xr0, xi1, fmaf(xi0, xr1, acc[3]));
+  }
+}
+// The same for NP polarisations held as complex values (NP = 1: power).
+template <int PMODE, int NP>
+__device__ __forceinline__ void detect_raw_np(float (&acc)[nq<PMODE>()], const float2 &x0,
+                                              const float2 &x1) {
+  if constexpr (NP == 2) {
+    detect_raw<PMODE>(acc, x0.x, x0.y, x1.x, x1.y);
+  } else {
+    static_assert(PMODE == 1, "one polarisation: total power only");
+    acc[0] = fmaf(x0.y, x0.y, fmaf(x0.x, x0.x, acc[0]));
+  }
+}
+template <int PMODE>
+__device__ __forceinline__ void detect_finish(float (&v)[nq<PMODE>()], float gam) {
+  if constexpr (PMODE != 1) {
+    if constexpr (PMODE == -4) {
+      const float xx = v[0], yy = v[1];
+      v[0] = xx + yy;
+      v[1] = xx - yy;
     }
-    acc[2] += re;
-    acc[3] += im;
+    v[3] *= gam;
   }
 }
 

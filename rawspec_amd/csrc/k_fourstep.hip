@@ -218,12 +218,11 @@ void k_fs_b(FsArgs a) {
     team_fft<N1, E, NP, TwTab<N1>::TWN>(v, lds + sg * team_stride<N1, NP>(), j, twp);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      float2 x[NP];
-#pragma unroll
-      for (int p = 0; p < NP; ++p) x[p] = v[p][m];
-      detect_add<PMODE, NP>(acc[m], x, a.gam);
+      detect_raw_np<PMODE, NP>(acc[m], v[0][m], v[NP - 1][m]);
     }
   }
+#pragma unroll
+  for (int m = 0; m < E; ++m) detect_finish<PMODE>(acc[m], a.gam);
 
   // internal accumulator block of this workgroup: [Q][N1][TB]; thread
   // element (q, m) at ib + (q*N1 + m*T)*TB.  The mode is uniform, so each

@@ -22,10 +22,11 @@ namespace rsk {
 constexpr int kMfmaWaves = 4;                      // k2 columns per workgroup
 constexpr int kOutRow = 4 * 1024 + 32;             // floats per q in the staging area
 
-// Detected powers of one k2 column and spectrum (both pols transformed,
-// then detected into acc, which is overwritten when SET, else added to).
+// Raw detection sums (detect_raw) of one k2 column and spectrum (both pols
+// transformed, then detected into acc, which is overwritten when SET, else
+// added to).
 template <int PMODE, bool SET>
-__device__ __forceinline__ void fs_b_column(const float4 *y, float s1, float gam,
+__device__ __forceinline__ void fs_b_column(const float4 *y, float s1,
                                             const Dft1024K &K, float (&acc)[16][nq<PMODE>()]) {
   const int lane = threadIdx.x & 63;
   const int col = lane & 31, h = lane >> 5;
@@ -52,8 +53,7 @@ __device__ __forceinline__ void fs_b_column(const float4 *y, float s1, float gam
 #pragma unroll
       for (int q = 0; q < nq<PMODE>(); ++q) acc[r][q] = 0.0f;
     }
-    const float2 xx[2] = {make_float2(xr[0][r], xi[0][r]), make_float2(xr[1][r], xi[1][r])};
-    detect_add<PMODE, 2>(acc[r], xx, gam);
+    detect_raw<PMODE>(acc[r], xr[0][r], xi[0][r], xr[1][r], xi[1][r]);
   }
 }
 
@@ -100,21 +100,24 @@ __global__ __launch_bounds__(64 * kMfmaWaves, ONE ? 2 : RSK_FSB_LOOP_WAVES) void
                      ((unsigned long long)cl * a.Nss * a.ynb + g * a.na_buf) * N +
                      (unsigned long long)tile * N1 * kMfmaWaves + wv;
   if constexpr (ONE) {
-    fs_b_column<PMODE, true>(y0, s1, a.gam, K, acc);
+    fs_b_column<PMODE, true>(y0, s1, K, acc);
   } else {
 #pragma unroll
     for (int r = 0; r < 16; ++r)
 #pragma unroll
       for (int q = 0; q < Q; ++q) acc[r][q] = 0.0f;
-    for (unsigned sp = 0; sp < a.na_buf; ++sp) fs_b_column<PMODE, false>(y0 + sp * N, s1, a.gam, K, acc);
+    for (unsigned sp = 0; sp < a.na_buf; ++sp) fs_b_column<PMODE, false>(y0 + sp * N, s1, K, acc);
   }
 
-  // undo the operand scaling (X = 32 * result / s1) and normalise
+  // raw sums -> components; undo the operand scaling (X = 32 * result /
+  // s1) and normalise
   const float sc = a.scale * (32.0f / s1) * (32.0f / s1);
 #pragma unroll
-  for (int r = 0; r < 16; ++r)
+  for (int r = 0; r < 16; ++r) {
+    detect_finish<PMODE>(acc[r], a.gam);
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[r][q] *= sc;
+  }
 
   // internal accumulator (integrations over several buffers): one
   // contiguous [Q][16][64] block per wavefront
@@ -226,11 +229,10 @@ __global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2
       dft1024<LO>(x, K, xr[p], xi[p]);
     }
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float2 xx[2] = {make_float2(xr[0][r], xi[0][r]), make_float2(xr[1][r], xi[1][r])};
-      detect_add<PMODE, 2>(acc[r], xx, a.gam);
-    }
+    for (int r = 0; r < 16; ++r) detect_raw<PMODE>(acc[r], xr[0][r], xi[0][r], xr[1][r], xi[1][r]);
   }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) detect_finish<PMODE>(acc[r], a.gam);
 
   // the transforms return X / 32: sums are 1024 x too small
   __syncthreads();   // every wavefront is done with the constants
@@ -258,6 +260,140 @@ __global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2
       dst[jo] = v;
     }
   }
+}
+
+// k_small_mfma_u: k_small_mfma's contract for int8 samples with unsplit
+// FP16 operands (mfma_dft.h, dft1024u): the samples become FP16 pairs by a
+// byte permute (no FP32 conversion), the inverse transform uses conjugated
+// constants (no sign flips per sample), detection accumulates raw sums with
+// fused multiply-adds and forms the components once per output.  The DFT
+// matrix (32 registers) stays in registers for the whole chunk, the
+// twiddles are read from LDS per transform; the next spectrum's samples are
+// loaded while the current one is transformed.
+template <int PMODE>
+__global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a, unsigned items) {
+  constexpr int N = 1024;
+  constexpr int Q = nq<PMODE>();
+  constexpr int kStBytes = kMfmaWaves * Q * kSmRow * 4;
+  constexpr int kLds = kStBytes > (int)sizeof(Dft1024U) ? kStBytes : (int)sizeof(Dft1024U);
+  __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLds];
+  Dft1024U &K = *reinterpret_cast<Dft1024U *>(lds_raw);
+  float *st = reinterpret_cast<float *>(lds_raw);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, h = lane >> 5;
+  const unsigned item = blockIdx.x * kMfmaWaves + wv;
+  const bool valid = item < items;
+  const unsigned it = valid ? item : 0u;
+  const unsigned c = it / (a.Ng * a.G);
+  const unsigned rest = it - c * (a.Ng * a.G);
+  const unsigned g = rest / a.G;
+  const unsigned chunk = rest - g * a.G;
+
+  // isign = -1 asks for the inverse transform: conjugated constants, and the
+  // cross term's sign is the input-conjugation sign alone
+  dft1024u_consts(K, a.tw, 64 * kMfmaWaves, a.isign);
+  __syncthreads();
+  h8v bre[4], bim[4];
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    bre[kb] = K.b[0][kb][lane];
+    bim[kb] = K.b[1][kb][lane];
+  }
+  const float gam = a.gam * a.isign;
+
+  float acc[16][Q];
+#pragma unroll
+  for (int r = 0; r < 16; ++r)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[r][q] = 0.0f;
+  const unsigned int *src = reinterpret_cast<const unsigned int *>(a.in + (unsigned long long)c * a.chan_stride) +
+                            ((unsigned long long)g * a.na_buf + (unsigned long long)chunk * a.chunk) * N +
+                            col + 32 * 4 * h;
+  const unsigned nspec = valid ? a.chunk : 0u;
+  unsigned int raw[16];
+  if (nspec) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) raw[e] = src[32 * (8 * (e >> 2) + (e & 3))];
+  }
+  for (unsigned s = 0; s < nspec; ++s) {
+    // FP16 (re, im) pairs of both pols: element (kb, i) is sample
+    // col + 32 (8 kb + 4 h + i)
+    h8v ah[2][4];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const unsigned int t = raw[e] ^ 0x80808080u;
+      const h2v p0 = __builtin_bit_cast(h2v, i8_pair_f16<0>(t));
+      const h2v p1 = __builtin_bit_cast(h2v, i8_pair_f16<1>(t));
+      ah[0][e >> 2][2 * (e & 3)] = p0[0];
+      ah[0][e >> 2][2 * (e & 3) + 1] = p0[1];
+      ah[1][e >> 2][2 * (e & 3)] = p1[0];
+      ah[1][e >> 2][2 * (e & 3) + 1] = p1[1];
+    }
+    if (s + 1 < nspec) {
+      const unsigned int *nx = src + (unsigned long long)(s + 1) * N;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) raw[e] = nx[32 * (8 * (e >> 2) + (e & 3))];
+    }
+    // the twiddles are read from LDS per transform (an opaque zero in the
+    // index keeps the compiler from holding all 32 registers of them)
+    int z;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    // one polarisation after the other (scheduling barriers): interleaving
+    // the two transforms would double their live registers
+    f16v xr0, xi0, xr1, xi1;
+    __builtin_amdgcn_sched_barrier(0);
+    dft1024u(ah[0], bre, bim, K.tw, lane + z, xr0, xi0);
+    __builtin_amdgcn_sched_barrier(0);
+    dft1024u(ah[1], bre, bim, K.tw, lane + z, xr1, xi1);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) detect_raw<PMODE>(acc[r], xr0[r], xi0[r], xr1[r], xi1[r]);
+  }
+
+  __syncthreads();   // every wavefront is done with the constants
+  float *sw = st + wv * (Q * kSmRow);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int k = 8 * (r >> 2) + 4 * h + (r & 3) + 32 * col;
+    const int jo = (k + N / 2) & (N - 1);            // FFT-shifted fine channel
+    detect_finish<PMODE>(acc[r], gam);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) sw[q * kSmRow + jo + (jo >> 5)] = acc[r][q];
+  }
+  __syncthreads();
+  if (!valid) return;
+  for (int q = 0; q < Q; ++q) {
+    float *dst;
+    if (a.direct) dst = a.acc + (((unsigned long long)g * Q + q) * a.Nc + c) * N;
+    else dst = a.part + ((((unsigned long long)c * a.Ng + g) * a.G + chunk) * Q + q) * N;
+#pragma unroll 4
+    for (int jo = lane; jo < N; jo += 64) {
+      float v = sw[q * kSmRow + jo + (jo >> 5)];
+      if (a.direct) {
+        v *= a.scale;
+        if (!a.first) v += dst[jo];
+      }
+      dst[jo] = v;
+    }
+  }
+}
+
+template <int PMODE>
+static int go_small_mfma_u(const SmallArgs &a, unsigned items, hipStream_t s) {
+  const unsigned nb = (items + kMfmaWaves - 1) / kMfmaWaves;
+  hipLaunchKernelGGL((k_small_mfma_u<PMODE>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a, items);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_small_mfma_u(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
+                        hipStream_t s) {
+  if (!small_mfma_ok(n, np) || nbps != 8 || items == 0 || a.gpw != 1) return 2;
+  switch (pmode) {
+    case 1: return go_small_mfma_u<1>(a, items, s);
+    case 4: return go_small_mfma_u<4>(a, items, s);
+    case -4: return go_small_mfma_u<-4>(a, items, s);
+  }
+  return 2;
 }
 
 template <int NBPS, int PMODE>

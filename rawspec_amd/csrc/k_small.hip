@@ -93,25 +93,37 @@ void k_small(SmallArgs a) {
 #pragma unroll
       for (int m = 0; m < E; ++m) raw[m] = src[sbase + j + m * T];
     }
+    // one thread per int8 transform: samples offset by 128, removed from bin
+    // 0 (the thread's element 0) after the single-stage transform
+    constexpr bool kOffset = T == 1 && NBPS == 8;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float2 o[NP];
-      S::cvt(raw[m], o, a.isign);
+      if constexpr (kOffset) S::cvt_offset(raw[m], o, a.isign);
+      else S::cvt(raw[m], o, a.isign);
 #pragma unroll
       for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
     // a team is T consecutive threads: within one wavefront when T <= 64
     team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp);
+    if constexpr (kOffset) {
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+        v[p][0].x -= 128.0f * N;
+        v[p][0].y -= a.isign * (128.0f * N);
+      }
+    }
     if (valid) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        float2 x[NP];
-#pragma unroll
-        for (int p = 0; p < NP; ++p) x[p] = v[p][m];
-        detect_add<PMODE, NP>(acc[m], x, a.gam);
+        detect_raw_np<PMODE, NP>(acc[m], v[0][m], v[NP - 1][m]);
       }
     }
   }
+  // raw sums -> the product's components (linear, so the slot sums below
+  // add finished values)
+#pragma unroll
+  for (int m = 0; m < E; ++m) detect_finish<PMODE>(acc[m], a.gam);
 
   __syncthreads();   // the FFT exchange area is reused
   float *base = a.direct ? a.acc : a.part;

@@ -110,6 +110,11 @@ int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
 int small_mfma_ok(int n, int np);
 int launch_small_mfma(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
                       hipStream_t s);
+// The same with unsplit FP16 operands (int8 samples only): a third of the
+// matrix instructions, error of order 6e-3 / sqrt(spectra integrated) over a
+// floor near 1.5e-4, for products that integrate enough spectra.
+int launch_small_mfma_u(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
+                        hipStream_t s);
 // Matrix-core pass B (N1 = 1024, two pols): a workgroup's 4 columns are one
 // k2 tile of Y (tby = fs_tile_b(1024) = 4).
 int fs_b_mfma_ok(int n1, int np);

@@ -141,4 +141,98 @@ __device__ __forceinline__ void dft1024(const float2 (&x)[4][4], const Dft1024K 
   dft32_stage<true>(ah, al, k, lane, xr, xi);
 }
 
+// ---- unsplit variant ------------------------------------------------------
+// The same two-stage DFT with every operand a single FP16 value: the 32-point
+// DFT matrix rounded once, int8 samples (exact in FP16) as they are, and the
+// twiddled stage-1 result rounded once to FP16 before stage 2 (FP32
+// accumulation in both stages).  A third of the matrix instructions of the
+// split form and no split arithmetic; each transform carries about 2^-12
+// relative rounding error, random from spectrum to spectrum except for the
+// matrix's own rounding, so an integration of K spectra has a worst bin error
+// of roughly 6e-3 / sqrt(K) over a floor near 1.5e-4 (measured on a numpy
+// emulation of these roundings with int8 noise and a strong tone).  Inputs are
+// used unscaled: |stage-1 result| <= 32 * 127 * sqrt(2) < 2^13.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+// Constants of the unsplit transform: B operands of the 32-point DFT matrix
+// (real-output and imaginary-output halves, 4 K-blocks) and the twiddles
+// W_1024^(a c) of each lane's 16 stage-1 results.  sg = -1 conjugates every
+// constant, which makes the transform the inverse DFT (unnormalised).
+struct Dft1024U {
+  h8v b[2][4][64];        // [re/im output][K-block][lane]
+  float2 tw[16][64];      // [stage-1 result register][lane]
+};
+
+__device__ __forceinline__ void dft1024u_consts(Dft1024U &k, const float2 *tw4096, int wg,
+                                                float sg) {
+  for (int e = threadIdx.x; e < 2 * 4 * 64; e += wg) {
+    const int lane = e & 63, kb = (e >> 6) & 3, t = e >> 8;
+    const int col = lane & 31, h = lane >> 5;
+    h8v H;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = 8 * kb + 4 * h + i;
+      const float2 w = tw4096[((b * col) & 31) * 128];
+      const float wx = w.x, wy = sg * w.y;
+      H[2 * i] = (_Float16)(t == 0 ? wx : wy);
+      H[2 * i + 1] = (_Float16)(t == 0 ? -wy : wx);
+    }
+    k.b[t][kb][lane] = H;
+  }
+  for (int e = threadIdx.x; e < 16 * 64; e += wg) {
+    const int lane = e & 63, r = e >> 6;
+    const int col = lane & 31, h = lane >> 5;
+    const int a = 8 * (r >> 2) + 4 * h + (r & 3);
+    const float2 w = tw4096[((a * col) & 1023) * 4];
+    k.tw[r][lane] = make_float2(w.x, sg * w.y);
+  }
+}
+
+// One unsplit 1024-point DFT.  ah: the lane's input samples as FP16 (re, im)
+// pairs (layout as dft1024); bre/bim: the B operands; tw: the twiddle table
+// (Dft1024U::tw, read at [r][lane]).
+// Returns X = DFT(x) in (xr, xi), register r = bin 8 (r / 4) + 4 h + r % 4 +
+// 32 (l % 32).
+__device__ __forceinline__ void dft1024u(const h8v (&ah)[4], const h8v (&bre)[4],
+                                         const h8v (&bim)[4], const float2 (*tw)[64], int lane,
+                                         f16v &xr, f16v &xi) {
+  f16v zr, zi;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { zr[r] = 0.0f; zi[r] = 0.0f; }
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    zr = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kb], bre[kb], zr, 0, 0, 0);
+    zi = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[kb], bim[kb], zi, 0, 0, 0);
+  }
+  h8v a2[4];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const float2 t = tw[r][lane];
+    const float wr = zr[r] * t.x - zi[r] * t.y;
+    const float wi = zr[r] * t.y + zi[r] * t.x;
+    a2[r >> 2][2 * (r & 3)] = (_Float16)wr;
+    a2[r >> 2][2 * (r & 3) + 1] = (_Float16)wi;
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { xr[r] = 0.0f; xi[r] = 0.0f; }
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    xr = __builtin_amdgcn_mfma_f32_32x32x16_f16(a2[kb], bre[kb], xr, 0, 0, 0);
+    xi = __builtin_amdgcn_mfma_f32_32x32x16_f16(a2[kb], bim[kb], xi, 0, 0, 0);
+  }
+}
+
+// int8 sample word (pol 0 re, im, pol 1 re, im) -> the (re, im) FP16 pair of
+// pol p, exact, with the SNORM clamp of -128 to -127: the bytes, offset by
+// 128, become the low bytes of FP16 values 1024 + u (exponent byte 0x64,
+// unit spacing), and 1152 is subtracted.
+template <int P>
+__device__ __forceinline__ unsigned int i8_pair_f16(unsigned int t) {   // t = word ^ 0x80808080
+  const unsigned int sel = P == 0 ? 0x04010400u : 0x04030402u;
+  h2v v = __builtin_bit_cast(h2v, __builtin_amdgcn_perm(0x64646464u, t, sel));
+  v = v - (h2v){(_Float16)1152.0f, (_Float16)1152.0f};
+  v = __builtin_elementwise_max(v, (h2v){(_Float16)-127.0f, (_Float16)-127.0f});
+  return __builtin_bit_cast(unsigned int, v);
+}
+
 }  // namespace rsk

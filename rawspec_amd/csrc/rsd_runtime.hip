@@ -57,6 +57,7 @@ struct ProdPlan {
   // matrix-core small kernel (N = 1024, two pols): one wavefront per chunk
   int m_ok;
   unsigned m_chunk, m_G;
+  int m_unsplit;           // int8 samples, long enough integration: unsplit FP16 operands allowed
   // four-step path
   unsigned N1, N2;
   unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
@@ -248,11 +249,20 @@ static void plan_small(rsd_ctx *d, ProdPlan &pp) {
   pp.bpc = ((pp.Ng + pp.gpw - 1) / pp.gpw) * pp.G;
 }
 
+// Shortest integration (spectra per dump) for which the matrix-core engine
+// at its 1e-3 bar (rsd_set_mfma level 1) uses unsplit FP16 operands: their
+// rounding error is random from spectrum to spectrum and falls as
+// 1/sqrt(spectra) from about 6e-3 for one spectrum to a floor near 1.5e-4
+// (3e-4 at 32 spectra in a numpy emulation of the roundings on int8 noise
+// with a strong tone).  Shorter integrations keep split operands.
+constexpr unsigned kUnsplitMinSpectra = 32;
+
 // Matrix-core small kernel plan: halve the chunks until t_mtarget
 // wavefronts (default 2048, eight per CU) are busy, keeping >= 8 spectra per
 // wavefront.
 static void plan_small_mfma(rsd_ctx *d, ProdPlan &pp) {
   pp.m_ok = rsk::small_mfma_ok((int)pp.p.Nt, (int)d->cfg.Np);
+  pp.m_unsplit = d->cfg.Nbps == 8 && pp.p.Na >= kUnsplitMinSpectra;
   pp.m_chunk = pp.na_buf;
   while ((unsigned long long)d->cfg.Nc * pp.Ng * (pp.na_buf / pp.m_chunk) < pp.t_mtarget &&
          pp.m_chunk % 2 == 0 && pp.m_chunk / 2 >= 8)
@@ -658,10 +668,17 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
       a.direct = pp.m_G == 1;
       const unsigned long long items = (unsigned long long)c.Nc * pp.Ng * pp.m_G;
       if (items > 0x7fffffffull) return 1;
-      ProfScope ps(d, i, "k_small_mfma", pp.p.Nt, s);
-      if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
-                                 (unsigned)items, s))
-        return 1;
+      if (d->mfma == 1 && pp.m_unsplit) {
+        ProfScope ps(d, i, "k_small_mfma_u", pp.p.Nt, s);
+        if (rsk::launch_small_mfma_u((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
+                                     (unsigned)items, s))
+          return 1;
+      } else {
+        ProfScope ps(d, i, "k_small_mfma", pp.p.Nt, s);
+        if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
+                                   (unsigned)items, s))
+          return 1;
+      }
     } else {
       const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
       if (nb > 0x7fffffffull) return 1;
@@ -1143,8 +1160,9 @@ int rsd_profile(rsd_ctx *d, int enable) {
   return 0;
 }
 
-int rsd_set_mfma(rsd_ctx *d, int enable) {
-  d->mfma = enable != 0;
+int rsd_set_mfma(rsd_ctx *d, int level) {
+  if (level < 0 || level > 2) return 1;
+  d->mfma = level;
   return 0;
 }
 

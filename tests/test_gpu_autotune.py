@@ -9,7 +9,7 @@ import re
 
 import pytest
 
-from tests.spectro_cases import MFMA_FLOOR_MULT, PLAIN_TOL, check_results, run_calls
+from tests.spectro_cases import MFMA_SPLIT, MFMA_FLOOR_MULT, PLAIN_TOL, check_results, run_calls
 
 pytestmark = pytest.mark.gpu
 
@@ -49,7 +49,7 @@ def test_autotune_mid_run_restarts_integrations():
 
 
 def test_autotune_matrix_core_engine():
-    kw = dict(Nc=2, Ntpb=1 << 20, Nts=[1 << 20, 1024], Nas=[2, 64], Npolout=[-4, -4], mfma=True,
+    kw = dict(Nc=2, Ntpb=1 << 20, Nts=[1 << 20, 1024], Nas=[2, 64], Npolout=[-4, -4], mfma=MFMA_SPLIT,
               tone=(1, 20.0, 300.0, 1024.0))
     r, cfg, got, reps = run_calls(["autotune", 2, 1, 1], **kw)
     _check_report(reps[0], kw["Nts"])

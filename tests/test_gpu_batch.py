@@ -5,7 +5,7 @@ integrations, on both transform engines."""
 import numpy as np
 import pytest
 
-from tests.spectro_cases import (MFMA_FLOOR_MULT, PLAIN_TOL, check_results, rel_err,
+from tests.spectro_cases import (MFMA_FLOOR_MULT, MFMA_SPLIT, PLAIN_TOL, check_results, rel_err,
                                  run_device_case)
 
 pytestmark = pytest.mark.gpu
@@ -20,7 +20,7 @@ def _run(**kw):
     return res, cfg
 
 
-@pytest.mark.parametrize("mfma", [False, True])
+@pytest.mark.parametrize("mfma", [0, MFMA_SPLIT])
 def test_batches_across_integration_boundaries(mfma):
     # Nis = 3 for the 2^20 product: batches [2, 3, 1, 3] start, continue
     # and complete integrations inside and at the edges of each batch; the
@@ -29,7 +29,7 @@ def test_batches_across_integration_boundaries(mfma):
          nbuf=9, batches=[2, 3, 1, 3], tone=(1, 20.0, 300.0, 1024.0), mfma=mfma)
 
 
-@pytest.mark.parametrize("mfma", [False, True])
+@pytest.mark.parametrize("mfma", [0, MFMA_SPLIT])
 def test_batch_covering_a_whole_integration(mfma):
     # one batch holding two whole integrations of 4 buffers (2 spectra each):
     # each sum goes straight to its dump

@@ -9,7 +9,7 @@ and mixed one-buffer / batch calls.
 import numpy as np
 import pytest
 
-from tests.spectro_cases import MFMA_FLOOR_MULT, PLAIN_TOL, check_results
+from tests.spectro_cases import MFMA_SPLIT, MFMA_FLOOR_MULT, PLAIN_TOL, check_results
 from tests.spectro_cases import run_calls as run_pattern
 
 pytestmark = pytest.mark.gpu
@@ -64,7 +64,7 @@ def test_held_several_spectra_per_buffer():
 
 
 def test_held_matrix_core_pass_b():
-    kw = dict(Nc=2, Ntpb=1 << 20, Nts=[1 << 20], Nas=[5], Npolout=[-4], mfma=True,
+    kw = dict(Nc=2, Ntpb=1 << 20, Nts=[1 << 20], Nas=[5], Npolout=[-4], mfma=MFMA_SPLIT,
               tone=(1, 20.0, 300.0, 1024.0))
     r1, cfg, g1, _ = run_pattern([1] * 10, **kw)
     check_results(r1, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT)

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from tests.spectro_cases import check_results, run_device_case
+from tests.spectro_cases import MFMA_SPLIT, check_results, run_device_case
 
 pytestmark = pytest.mark.gpu
 
@@ -49,8 +49,8 @@ def test_lanes_match_one_stream(case, batches):
 
 
 def test_lanes_matrix_engine():
-    r1, cfg, e1 = _run("mixed", True, [4, 4], mfma=True)
-    r0, _, e0 = _run("mixed", False, [4, 4], mfma=True)
+    r1, cfg, e1 = _run("mixed", True, [4, 4], mfma=MFMA_SPLIT)
+    r0, _, e0 = _run("mixed", False, [4, 4], mfma=MFMA_SPLIT)
     assert e1 == e0
     for a, b in zip(r1, r0):
         assert np.array_equal(a[2], b[2])
