@@ -31,7 +31,8 @@ RAWSPEC_CALLBACK_PRE_DUMP = 0
 RAWSPEC_CALLBACK_POST_DUMP = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "librawspec.so")
+# RAWSPEC_AMD_LIB names another build of the library (e.g. a sanitizer build)
+_LIB_PATH = os.environ.get("RAWSPEC_AMD_LIB") or os.path.join(_HERE, "librawspec.so")
 
 
 class RawspecContext(C.Structure):

@@ -50,34 +50,60 @@ def _run(cmd):
     return r
 
 
-def compile_one(src, build_dir=BUILD, defines=()):
+SAN = os.path.join(HERE, "_san")
+
+
+def san_flags(sanitize):
+    """gcc flags of a sanitizer build ("address,undefined" or "thread"):
+    every finding aborts the process."""
+    if not sanitize:
+        return []
+    return [f"-fsanitize={sanitize}", "-fno-omit-frame-pointer", "-g",
+            "-fno-sanitize-recover=all"]
+
+
+def san_libs(sanitize):
+    """gcc's sanitizer runtimes, linked into a library whose host C is
+    instrumented (the HIP objects are not)."""
+    libs = []
+    for kind, name in (("address", "libasan.so"), ("undefined", "libubsan.so"),
+                       ("thread", "libtsan.so")):
+        if sanitize and kind in sanitize.split(","):
+            libs.append(subprocess.run([CC, f"-print-file-name={name}"], capture_output=True,
+                                       text=True, check=True).stdout.strip())
+    return libs
+
+
+def compile_one(src, build_dir=BUILD, defines=(), sanitize=None):
     path = os.path.join(CSRC, src)
     out = os.path.join(build_dir, os.path.basename(src) + ".o")
     dflags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
         cmd = [HIPCC, "-c", *HIP_FLAGS, *dflags, "-x", "hip", path, "-o", out]
     else:
-        cmd = [CC, "-c", *C_FLAGS, *dflags, path, "-o", out]
+        cmd = [CC, "-c", *C_FLAGS, *san_flags(sanitize), *dflags, path, "-o", out]
     _run(cmd)
     return out
 
 
 def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
-          defines=(), build_dir: str = BUILD) -> str:
+          defines=(), build_dir: str = BUILD, sanitize: str | None = None) -> str:
     """Compile every source and link librawspec.so; returns its path.
 
     defines: extra preprocessor definitions (the kernels' compile-time
-    tunables, e.g. "RSK_E_BIG=8"); out/build_dir redirect the result."""
+    tunables, e.g. "RSK_E_BIG=8"); out/build_dir redirect the result;
+    sanitize: instrument the host C (gcc -fsanitize=...; see sanitize.sh
+    beside the tests)."""
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     os.makedirs(build_dir, exist_ok=True)
     srcs = HIP_SOURCES + C_SOURCES
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, build_dir, defines), srcs))
+        objs = list(ex.map(lambda s: compile_one(s, build_dir, defines, sanitize), srcs))
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
-          "-Wl,--no-undefined", "-Wl,-soname,librawspec.so"])
+          *san_libs(sanitize), "-Wl,--no-undefined", "-Wl,-soname,librawspec.so"])
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")
@@ -110,22 +136,25 @@ def build_fbh5_plugin(out_dir: str, verbose: bool = False):
     return out
 
 
-def build_cli(lib: str = LIB, out: str = CLI, verbose: bool = False) -> str:
+def build_cli(lib: str = LIB, out: str = CLI, verbose: bool = False, sanitize: str | None = None,
+              build_dir: str = BUILD) -> str:
     """Compile the rawspec program (host C) against the built library; it
     finds librawspec.so next to its own directory at run time."""
     os.makedirs(os.path.dirname(out), exist_ok=True)
+    os.makedirs(build_dir, exist_ok=True)
     objs = []
     for src in CLI_SOURCES:
-        o = os.path.join(BUILD, "cli_" + src + ".o")
-        _run([CC, "-c", *C_FLAGS, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", f"-I{CLI_SRC}",
-              os.path.join(CLI_SRC, src), "-o", o])
+        o = os.path.join(build_dir, "cli_" + src + ".o")
+        _run([CC, "-c", *C_FLAGS, *san_flags(sanitize), "-D__HIP_PLATFORM_AMD__",
+              f"-I{ROCM}/include", f"-I{CLI_SRC}", os.path.join(CLI_SRC, src), "-o", o])
         objs.append(o)
     tmp = out + ".tmp"
-    _run([CC, "-o", tmp, *objs, f"-L{os.path.dirname(lib)}", "-lrawspec",
+    _run([CC, *san_flags(sanitize), "-o", tmp, *objs, f"-L{os.path.dirname(lib)}", "-lrawspec",
           "-Wl,-rpath,$ORIGIN/..", f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
           "-lpthread", "-ldl", "-lm"])
     os.replace(tmp, out)
-    build_fbh5_plugin(os.path.dirname(out), verbose)
+    if not sanitize:
+        build_fbh5_plugin(os.path.dirname(out), verbose)
     if verbose:
         print(f"built {out}")
     return out
@@ -137,11 +166,28 @@ def exported_symbols(path: str = LIB):
     return sorted(line.split()[-1] for line in r.stdout.splitlines() if " T " in line)
 
 
+def build_sanitized(sanitize: str, verbose: bool = False):
+    """librawspec.so and the rawspec program with gcc's sanitizers on the
+    host C, under rawspec_amd/_san/<first kind>/ (the normal build is left
+    alone); returns (library, program)."""
+    d = os.path.join(SAN, sanitize.split(",")[0])
+    lib = build(verbose=verbose, out=os.path.join(d, "librawspec.so"),
+                build_dir=os.path.join(d, "_build"), sanitize=sanitize)
+    cli = build_cli(lib=lib, out=os.path.join(d, "bin", "rawspec"), verbose=verbose,
+                    sanitize=sanitize, build_dir=os.path.join(d, "_build"))
+    return lib, cli
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--check", action="store_true")
     ap.add_argument("-j", type=int, default=None)
+    ap.add_argument("--sanitize", default=None,
+                    help="address,undefined or thread: an instrumented build under rawspec_amd/_san/")
     a = ap.parse_args(argv)
+    if a.sanitize:
+        build_sanitized(a.sanitize, verbose=True)
+        return 0
     build(a.j, verbose=True)
     build_cli(verbose=True)
     if a.check:

@@ -15,7 +15,8 @@ from tests.raw_stems import StemSpec, block_data, pktidx_of, read_fil, seq_of, w
 from tests.spectro_cases import PLAIN_TOL, rel_err
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CLI = os.path.join(ROOT, "rawspec_amd", "bin", "rawspec")
+# RAWSPEC_AMD_CLI names another build of the program (e.g. a sanitizer build)
+CLI = os.environ.get("RAWSPEC_AMD_CLI") or os.path.join(ROOT, "rawspec_amd", "bin", "rawspec")
 
 
 def run_cli(args, timeout=300, check=True):

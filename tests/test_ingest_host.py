@@ -21,7 +21,11 @@ HARNESS_DIR = os.path.join(ROOT, "tests", "ingest_harness")
 @pytest.fixture(scope="module")
 def harness(tmp_path_factory):
     out = str(tmp_path_factory.mktemp("ingest") / "harness")
-    subprocess.run(["gcc", "-O2", "-std=gnu11", "-pthread", "-Wall", f"-I{HARNESS_DIR}",
+    # RAWSPEC_SANITIZE (e.g. "address,undefined" or "thread") instruments it
+    san = os.environ.get("RAWSPEC_SANITIZE")
+    sflags = [f"-fsanitize={san}", "-fno-omit-frame-pointer", "-g",
+              "-fno-sanitize-recover=all"] if san else []
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-pthread", "-Wall", *sflags, f"-I{HARNESS_DIR}",
                     f"-I{ROOT}/include", f"-I{ROOT}/rawspec_amd/csrc/cli",
                     f"{ROOT}/rawspec_amd/csrc/cli/cli_ingest.c", f"{HARNESS_DIR}/harness.c",
                     "-o", out], check=True)
