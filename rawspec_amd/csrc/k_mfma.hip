@@ -268,8 +268,8 @@ __global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2
 // constants (no sign flips per sample), detection accumulates raw sums with
 // fused multiply-adds and forms the components once per output.  The DFT
 // matrix (32 registers) stays in registers for the whole chunk, the
-// twiddles are read from LDS per transform; the next spectrum's samples are
-// loaded while the current one is transformed.
+// twiddles are read from LDS per transform; the samples are loaded two
+// spectra ahead of the transforms.
 template <int PMODE>
 __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a, unsigned items) {
   constexpr int N = 1024;
@@ -310,10 +310,16 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
                             ((unsigned long long)g * a.na_buf + (unsigned long long)chunk * a.chunk) * N +
                             col + 32 * 4 * h;
   const unsigned nspec = valid ? a.chunk : 0u;
-  unsigned int raw[16];
+  // samples of spectrum s in raw, of s + 1 in nxt (loads two spectra ahead
+  // of the transforms: one spectrum's compute does not cover HBM latency)
+  unsigned int raw[16], nxt[16];
   if (nspec) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) raw[e] = src[32 * (8 * (e >> 2) + (e & 3))];
+  }
+  if (nspec > 1) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) nxt[e] = src[N + 32 * (8 * (e >> 2) + (e & 3))];
   }
   for (unsigned s = 0; s < nspec; ++s) {
     // FP16 (re, im) pairs of both pols: element (kb, i) is sample
@@ -329,10 +335,12 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
       ah[1][e >> 2][2 * (e & 3)] = p1[0];
       ah[1][e >> 2][2 * (e & 3) + 1] = p1[1];
     }
-    if (s + 1 < nspec) {
-      const unsigned int *nx = src + (unsigned long long)(s + 1) * N;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) raw[e] = nx[32 * (8 * (e >> 2) + (e & 3))];
+    for (int e = 0; e < 16; ++e) raw[e] = nxt[e];
+    if (s + 2 < nspec) {
+      const unsigned int *nx = src + (unsigned long long)(s + 2) * N;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) nxt[e] = nx[32 * (8 * (e >> 2) + (e & 3))];
     }
     // the twiddles are read from LDS per transform (an opaque zero in the
     // index keeps the compiler from holding all 32 registers of them)

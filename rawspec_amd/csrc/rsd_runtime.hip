@@ -734,7 +734,12 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   // The engine is chosen when an integration starts and kept until it
   // completes (the two pass-B kernels lay out the internal accumulator
   // differently).
-  if (first) pp.fs_mfma = d->mfma && rsk::fs_b_mfma_ok((int)pp.N1, (int)c.Np);
+  // Level 1 keeps pass B on the vector units: the matrix-core pass B must
+  // split every operand (the tone of a four-step product sits coherently in
+  // a row of the intermediate, so unsplit FP16 would miss the 1e-3 bar) and
+  // is then slower than the vector pass B (0.358 against 0.256 ms per 256 MiB
+  // buffer of the 64-channel 2^20 product); level 2 runs it.
+  if (first) pp.fs_mfma = d->mfma == 2 && rsk::fs_b_mfma_ok((int)pp.N1, (int)c.Np);
   // both pass-B kernels read Y tiled by fs_tile_b(N1) = 4 consecutive k2
   // (for N1 = 1024, the matrix-core pass B's workgroup of 4 columns)
   a.tby = (unsigned)rsk::fs_tile_b((int)pp.N1);
