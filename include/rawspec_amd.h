@@ -87,6 +87,16 @@ float *rawspec_amd_device_icsbuf(rawspec_context *ctx, int i);
  * own device work after the context's. */
 void *rawspec_amd_stream(rawspec_context *ctx);
 
+/* Queues the context's work (kernels, dump copies, callbacks) on `stream`, a
+ * hipStream_t of the context's GPU, from now on; NULL returns to the
+ * context's own stream.  Work already queued is ordered before the work that
+ * follows on the new stream.  Several contexts may share one stream (a
+ * server keeps its streams within the GPU's hardware queues that way);
+ * rawspec_wait_for_completion() then also waits for the other contexts' work
+ * on it.  The caller keeps the stream alive while the context uses it.
+ * Returns 0 on success. */
+int rawspec_amd_set_stream(rawspec_context *ctx, void *stream);
+
 /* Transform engine of the context.  enable = 0: FP32 on the vector units
  * (the default).  enable = 1: the matrix cores (FP16 MFMA with FP32
  * accumulation) wherever a matrix-core kernel exists for the product's shape,

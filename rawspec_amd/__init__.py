@@ -207,6 +207,7 @@ def lib():
             ("rawspec_amd_device_pwrbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_device_icsbuf", [P, C.c_int], C.c_void_p),
             ("rawspec_amd_stream", [P], C.c_void_p),
+            ("rawspec_amd_set_stream", [P, C.c_void_p], C.c_int),
             ("rawspec_amd_profile", [P, C.c_int], C.c_int),
             ("rawspec_amd_set_mfma", [P, C.c_int], C.c_int),
             ("rawspec_amd_get_mfma", [P], C.c_int),
@@ -330,6 +331,10 @@ class Spectrometer:
 
     def stream(self) -> int:
         return lib().rawspec_amd_stream(C.byref(self.ctx)) or 0
+
+    def set_stream(self, stream: int) -> int:
+        """Queue this context's work on the hipStream_t `stream` (0: its own)."""
+        return lib().rawspec_amd_set_stream(C.byref(self.ctx), C.c_void_p(stream or None))
 
     def set_mfma(self, level) -> int:
         """0 / False: vector units; 1 / True: matrix cores at the 1e-3 bar

@@ -455,6 +455,29 @@ __device__ __forceinline__ float2 load_y(const float2 *p) {
   }
 }
 
+// 24-bit intermediate (engine level 1, two pols): each float of Y is rounded
+// to nearest even at mantissa bit 8 and its upper three bytes kept, so a
+// two-pol complex value takes 12 bytes instead of 16 (every value keeps its
+// own exponent; 16 significant bits).  Packing and unpacking are byte
+// permutes.
+__device__ __forceinline__ unsigned int round24(float x) {
+  const unsigned int b = __float_as_uint(x);
+  return b + 0x7fu + ((b >> 8) & 1u);   // the upper three bytes hold the rounded value
+}
+__device__ __forceinline__ void store_y24(unsigned int *p, float4 v) {
+  const unsigned int b0 = round24(v.x), b1 = round24(v.y), b2 = round24(v.z), b3 = round24(v.w);
+  __builtin_nontemporal_store(__builtin_amdgcn_perm(b1, b0, 0x05030201u), p);
+  __builtin_nontemporal_store(__builtin_amdgcn_perm(b2, b1, 0x06050302u), p + 1);
+  __builtin_nontemporal_store(__builtin_amdgcn_perm(b3, b2, 0x07060503u), p + 2);
+}
+__device__ __forceinline__ float4 load_y24(const unsigned int *p) {
+  const unsigned int w0 = __builtin_nontemporal_load(p), w1 = __builtin_nontemporal_load(p + 1),
+                     w2 = __builtin_nontemporal_load(p + 2);
+  return make_float4(__uint_as_float(w0 << 8), __uint_as_float(__builtin_amdgcn_perm(w1, w0, 0x0504030cu)),
+                     __uint_as_float(__builtin_amdgcn_perm(w2, w1, 0x0403020cu)),
+                     __uint_as_float(w2 & 0xffffff00u));
+}
+
 // The running sum of an integration in its internal layout: each block is
 // read and written once per pass-B launch by the one workgroup (or
 // wavefront) that owns it, so RSK_NT_ACC = 1 streams it past the caches

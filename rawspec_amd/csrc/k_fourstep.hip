@@ -66,8 +66,9 @@ __device__ __forceinline__ unsigned y_off(unsigned j, unsigned n1, int m) {
 // N1 entries of the FP64-generated fine table; so each twiddle costs two LDS
 // reads and one complex product (two correctly rounded factors, as with the
 // global two-level table) instead of two scattered global gathers.
-template <int N2, int R1, int NBPS, int NP>
+template <int N2, int R1, int NBPS, int NP, bool Y24>
 __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
+  static_assert(!Y24 || NP == 2, "24-bit intermediate: two pols");
   constexpr int E = elems_fs(N2);
   constexpr int T = N2 / E;
   constexpr int TA = tile_of(N2);      // n1 values per workgroup
@@ -142,7 +143,9 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
     const unsigned cl = cb / a.nbat;
     const unsigned n1 = tile * TA + sg;
     const unsigned long long spec = (unsigned long long)cl * nsp + (a.yb0 + b) * a.Nss + sp;
-    float2 *dst = a.work + spec * N * NP;
+    // spectrum `spec` starts at N * NP float2 (24-bit: N * 12 bytes) into Y
+    float2 *dst = Y24 ? reinterpret_cast<float2 *>(reinterpret_cast<unsigned int *>(a.work) + spec * N * 3)
+                      : a.work + spec * N * NP;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const unsigned k2 = j + m * T;
@@ -151,7 +154,9 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
       const unsigned off = y_off<T, TBY, N1C>(j, n1, m);
       if constexpr (NP == 2) {
         const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
-        store_y(reinterpret_cast<float4 *>(dst) + off, make_float4(y0.x, y0.y, y1.x, y1.y));
+        const float4 yv = make_float4(y0.x, y0.y, y1.x, y1.y);
+        if constexpr (Y24) store_y24(reinterpret_cast<unsigned int *>(dst) + 3ull * off, yv);
+        else store_y(reinterpret_cast<float4 *>(dst) + off, yv);
       } else {
         store_y(dst + off, cmul(v[0][m], w));
       }
@@ -168,9 +173,10 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
 #ifndef RSK_FSB_WAVES
 #define RSK_FSB_WAVES 2
 #endif
-template <int N1, int NP, int PMODE>
+template <int N1, int NP, int PMODE, bool Y24>
 __global__ __launch_bounds__(wg_team(N1)) __attribute__((amdgpu_waves_per_eu(RSK_FSB_WAVES)))
 void k_fs_b(FsArgs a) {
+  static_assert(!Y24 || NP == 2, "24-bit intermediate: two pols");
   constexpr int E = elems_fs(N1);
   constexpr int T = N1 / E;
   constexpr int TB = tile_of(N1);      // k2 values per workgroup
@@ -201,18 +207,22 @@ void k_fs_b(FsArgs a) {
 
   for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
     // this workgroup's tile: N1 rows of TB k2 values, contiguous
-    const float2 *y = a.work + (((unsigned long long)cl * a.Nss * a.ynb + sp) * N +
-                                (unsigned long long)tile * N1 * TB) * NP;
+    const unsigned long long e0 = ((unsigned long long)cl * a.Nss * a.ynb + sp) * N +
+                                  (unsigned long long)tile * N1 * TB;   // first value of the tile
+    const float2 *y = a.work + e0 * NP;
+    const unsigned int *y3 = reinterpret_cast<const unsigned int *>(a.work) + e0 * 3;
     float2 v[NP][E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      const unsigned long long off = ((unsigned long long)(j + m * T) * TB + sg) * NP;
+      const unsigned long long ev = (unsigned long long)(j + m * T) * TB + sg;
       if constexpr (NP == 2) {
-        const float4 t = load_y(reinterpret_cast<const float4 *>(y + off));
+        float4 t;
+        if constexpr (Y24) t = load_y24(y3 + 3 * ev);
+        else t = load_y(reinterpret_cast<const float4 *>(y + ev * NP));
         v[0][m] = make_float2(t.x, t.y);
         v[1][m] = make_float2(t.z, t.w);
       } else {
-        v[0][m] = load_y(y + off);
+        v[0][m] = load_y(y + ev * NP);
       }
     }
     team_fft<N1, E, NP, TwTab<N1>::TWN>(v, lds + sg * team_stride<N1, NP>(), j, twp);
@@ -267,8 +277,9 @@ int fs_tile_b(int n1) { return (int)(wg_team(n1) / (n1 / elems_fs(n1))); }
 template <int N2, int R1, int NBPS>
 static int go_a_r(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
   constexpr int WGT = wg_team(N2);
-  if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2>), dim3(nb), dim3(WGT), 0, s, a);
-  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1>), dim3(nb), dim3(WGT), 0, s, a);
+  if (np == 2 && a.y24) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2, true>), dim3(nb), dim3(WGT), 0, s, a);
+  else if (np == 2) hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 2, false>), dim3(nb), dim3(WGT), 0, s, a);
+  else hipLaunchKernelGGL((k_fs_a<N2, R1, NBPS, 1, false>), dim3(nb), dim3(WGT), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -296,6 +307,7 @@ static int go_a_n(int np, const FsArgs &a, unsigned nb, hipStream_t s) {
 }
 
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
+  if (a.y24 && np != 2) return 2;
   const unsigned tiles = a.N1 / fs_tile_a((int)a.N2);
   const unsigned long long items = (unsigned long long)tiles * a.Nss * a.nbat * a.nch;
   const unsigned long long nb = a.grid_a && a.grid_a < items ? a.grid_a : items;
@@ -308,17 +320,22 @@ int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s) {
   return 2;
 }
 
+template <int N1, bool Y24>
+static void go_b2(int pmode, const FsArgs &a, unsigned nb, hipStream_t s) {
+  constexpr int WGT = wg_team(N1);
+  if (pmode == 1) hipLaunchKernelGGL((k_fs_b<N1, 2, 1, Y24>), dim3(nb), dim3(WGT), 0, s, a);
+  else if (pmode == 4) hipLaunchKernelGGL((k_fs_b<N1, 2, 4, Y24>), dim3(nb), dim3(WGT), 0, s, a);
+  else hipLaunchKernelGGL((k_fs_b<N1, 2, -4, Y24>), dim3(nb), dim3(WGT), 0, s, a);
+}
+
 template <int N1>
 static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) {
   constexpr int WGT = wg_team(N1);
   if (np == 1) {
-    hipLaunchKernelGGL((k_fs_b<N1, 1, 1>), dim3(nb), dim3(WGT), 0, s, a);
-  } else if (pmode == 1) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, 1>), dim3(nb), dim3(WGT), 0, s, a);
-  } else if (pmode == 4) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, 4>), dim3(nb), dim3(WGT), 0, s, a);
-  } else if (pmode == -4) {
-    hipLaunchKernelGGL((k_fs_b<N1, 2, -4>), dim3(nb), dim3(WGT), 0, s, a);
+    hipLaunchKernelGGL((k_fs_b<N1, 1, 1, false>), dim3(nb), dim3(WGT), 0, s, a);
+  } else if (pmode == 1 || pmode == 4 || pmode == -4) {
+    if (a.y24) go_b2<N1, true>(pmode, a, nb, s);
+    else go_b2<N1, false>(pmode, a, nb, s);
   } else {
     return 2;
   }
@@ -326,6 +343,7 @@ static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) 
 }
 
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
+  if (a.y24 && np != 2) return 2;
   const unsigned tiles = a.N2 / fs_tile_b((int)a.N1);
   const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.nch;
   if (nb == 0 || nb > 0x7fffffffull || a.nbat == 0 || a.nbat > a.ynb) return 2;

@@ -444,7 +444,7 @@ static int go_b_mfma(const FsArgs &a, unsigned nb, hipStream_t s) {
 int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
 
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {
-  if (!fs_b_mfma_ok((int)a.N1, np) || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f) ||
+  if (!fs_b_mfma_ok((int)a.N1, np) || a.y24 || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f) ||
       a.nbat == 0 || a.nbat > a.ynb || a.tby != (unsigned)kMfmaWaves)
     return 2;
   const unsigned long long nb = (unsigned long long)(a.N2 / kMfmaWaves) * a.Ng * a.nch;

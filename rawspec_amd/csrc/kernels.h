@@ -76,6 +76,7 @@ struct FsArgs {
   unsigned grid_a;                // pass-A workgroups, each looping over items (0: one per item)
   unsigned tby;                   // k2 tile of Y: fs_tile_b(N1), both pass-B kernels
   float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
+  int y24;                        // two pols: Y holds 12-byte (24-bit float) values
 };
 
 struct ReduceArgs {

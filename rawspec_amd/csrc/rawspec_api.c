@@ -641,6 +641,13 @@ void *rawspec_amd_stream(rawspec_context *ctx)
   return (g && g->dev) ? rsd_stream(g->dev) : NULL;
 }
 
+int rawspec_amd_set_stream(rawspec_context *ctx, void *stream)
+{
+  rs_gpu_ctx *g = rs_get(ctx);
+  if (!g) return 1;
+  return rsd_set_stream(g->dev, stream);
+}
+
 int rawspec_amd_profile(rawspec_context *ctx, int enable)
 {
   rs_gpu_ctx *g = rs_get(ctx);

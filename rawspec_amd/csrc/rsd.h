@@ -98,6 +98,11 @@ size_t rsd_input_bytes(rsd_ctx *d);
 float *rsd_pwr_ptr(rsd_ctx *d, int i);
 float *rsd_ics_ptr(rsd_ctx *d, int i);
 void  *rsd_stream(rsd_ctx *d);
+/* Queues the context's work on `stream` (a hipStream_t of the context's
+ * device) from now on, after everything already queued; NULL returns to
+ * the context's own stream.  The caller keeps the stream alive while the
+ * context uses it. */
+int    rsd_set_stream(rsd_ctx *d, void *stream);
 
 /* Human-readable name of the FFT engine (returned by get_cufft_version). */
 const char *rsd_engine_name(void);

@@ -63,6 +63,7 @@ struct ProdPlan {
   unsigned fs_chunk;       // channels per pass A/B launch pair (one buffer)
   unsigned fs_bmax;        // buffers per batched launch pair (1: no batching)
   int fs_mfma;             // pass B on the matrix cores for the current integration
+  int fs_y24;              // 24-bit intermediate for the current integration
   // launch-plan tunables (defaults below; rsd_autotune times alternatives)
   unsigned t_target;       // small kernel: workgroups a launch should reach
   unsigned t_maxspg;       // small kernel: most slots per dump group
@@ -128,7 +129,8 @@ std::vector<float2> twiddles(unsigned long long n, unsigned long long cnt, unsig
 
 struct rsd_ctx {
   int dev;
-  hipStream_t stream;
+  hipStream_t stream;                  // the stream the context's work goes to
+  hipStream_t own_stream;              // created with the context (rsd_set_stream may replace `stream`)
   // Products run on two lanes forked from `stream` and joined back into it:
   // four-step products (which share d_work) on lane 0, small products (which
   // share d_part) on lane 1, so the vector-bound small transforms overlap
@@ -381,7 +383,10 @@ void rsd_destroy(rsd_ctx *d) {
     if (d->ev_lane[k]) (void)hipEventDestroy(d->ev_lane[k]);
   }
   if (d->ev_main) (void)hipEventDestroy(d->ev_main);
-  if (d->stream) (void)hipStreamDestroy(d->stream);
+  if (d->own_stream) {
+    (void)hipStreamSynchronize(d->own_stream);
+    (void)hipStreamDestroy(d->own_stream);
+  }
   delete d;
 }
 
@@ -538,7 +543,8 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
   }
   if (alloc_zero((void **)&d->d_part, d->part_len * sizeof(float))) return 1;
   if (alloc_zero((void **)&d->d_work, d->work_len * sizeof(float2))) return 1;
-  RSD_CHECK(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  RSD_CHECK(hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
+  d->stream = d->own_stream;
   RSD_CHECK(hipEventCreateWithFlags(&d->ev_main, hipEventDisableTiming));
   for (int k = 0; k < 2; ++k) {
     RSD_CHECK(hipStreamCreateWithFlags(&d->lane[k], hipStreamNonBlocking));
@@ -556,7 +562,7 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
 static rsd_ctx *new_ctx() {
   rsd_ctx *d = new rsd_ctx();
   memset(d->pp, 0, sizeof(d->pp));
-  d->stream = nullptr;
+  d->stream = d->own_stream = nullptr;
   d->lane[0] = d->lane[1] = nullptr;
   d->ev_main = d->ev_lane[0] = d->ev_lane[1] = nullptr;
   d->lanes = false;
@@ -740,6 +746,13 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   // is then slower than the vector pass B (0.358 against 0.256 ms per 256 MiB
   // buffer of the 64-channel 2^20 product); level 2 runs it.
   if (first) pp.fs_mfma = d->mfma == 2 && rsk::fs_b_mfma_ok((int)pp.N1, (int)c.Np);
+  // Level 1 also keeps the intermediate in 24-bit floats (16 significant
+  // bits, each value its own exponent): a quarter less of the traffic that
+  // bounds both passes, for at most about 2e-5 of added error at one
+  // spectrum per dump (numpy model of the 2^20 product with a strong tone;
+  // it falls as the integration grows), fifty times under the 1e-3 bar.
+  if (first) pp.fs_y24 = d->mfma == 1 && c.Np == 2 && !pp.fs_mfma;
+  a.y24 = pp.fs_y24;
   // both pass-B kernels read Y tiled by fs_tile_b(N1) = 4 consecutive k2
   // (for N1 = 1024, the matrix-core pass B's workgroup of 4 columns)
   a.tby = (unsigned)rsk::fs_tile_b((int)pp.N1);
@@ -1159,6 +1172,18 @@ size_t rsd_input_bytes(rsd_ctx *d) { return d->in_bytes; }
 float *rsd_pwr_ptr(rsd_ctx *d, int i) { return d->pp[i].d_acc; }
 float *rsd_ics_ptr(rsd_ctx *d, int i) { return d->pp[i].d_ics; }
 void *rsd_stream(rsd_ctx *d) { return (void *)d->stream; }
+
+int rsd_set_stream(rsd_ctx *d, void *stream) {
+  hipStream_t s = stream ? (hipStream_t)stream : d->own_stream;
+  if (s == d->stream) return 0;
+  RSD_CHECK(hipSetDevice(d->dev));
+  // work already queued on the old stream precedes everything queued on the
+  // new one (the integration buffers carry over)
+  RSD_CHECK(hipEventRecord(d->ev_main, d->stream));
+  RSD_CHECK(hipStreamWaitEvent(s, d->ev_main, 0));
+  d->stream = s;
+  return 0;
+}
 
 int rsd_profile(rsd_ctx *d, int enable) {
   d->prof = enable != 0;

@@ -194,7 +194,14 @@ class _Stream:
         if self.sp.set_lanes(srv.lanes):
             raise RuntimeError("rawspec_amd_set_lanes failed")
         assert self.sp.device_input_size() == plan_.buf_bytes
-        self.hs = torch.cuda.ExternalStream(self.sp.stream(), device=f"cuda:{srv.gpu}")
+        pool = srv.stream_pool()
+        if pool:
+            # contexts share the server's few streams (rawspec_amd_set_stream)
+            self.hs = pool[srv.take_slot() % len(pool)]
+            if self.sp.set_stream(self.hs.cuda_stream):
+                raise RuntimeError("rawspec_amd_set_stream failed")
+        else:
+            self.hs = torch.cuda.ExternalStream(self.sp.stream(), device=f"cuda:{srv.gpu}")
         self.ring = srv.make_ring(req, plan_)
 
     def _on_dump(self, sp, product, cbtype):
@@ -253,7 +260,9 @@ class Server:
     submission (1: rawspec_amd_start_processing_device, more: the batched
     entry point).  mfma: transform engine of every stream's context.  ring:
     distinct device input buffers per stream (buffer k reads
-    ring[k % ring]).  stream_cls: the
+    ring[k % ring]).  streams: HIP streams the contexts share (assigned in
+    turn at admission, rawspec_amd_set_stream); 0 gives every context its
+    own stream.  stream_cls: the
     class that serves one admitted request (its constructor, pump(),
     finished, close(), sent, lat, dumps, plan); the scheduler itself touches
     no device.
@@ -261,14 +270,29 @@ class Server:
 
     def __init__(self, hbm_budget=200e9, host_budget=96 * GiB, max_active=32, depth=2, ring=2,
                  gpu=0, record_dumps=False, poll_s=20e-6, stream_cls=None, batch=1, mfma=False,
-                 lanes=False):
+                 lanes=False, streams=0):
         self.hbm_budget, self.host_budget = hbm_budget, host_budget
         self.max_active, self.depth, self.nring = max_active, depth, ring
-        self.batch, self.mfma = max(1, int(batch)), bool(mfma)
+        self.batch, self.mfma = max(1, int(batch)), int(mfma)
+        self.nstreams, self._pool, self._slot = int(streams), None, 0
         self.lanes = bool(lanes)
         self.gpu, self.record_dumps, self.poll_s = gpu, record_dumps, poll_s
         self.stream_cls = stream_cls or _Stream
         self._gen = None
+
+    def stream_pool(self):
+        """The shared streams (default priority), created on first use."""
+        if self.nstreams <= 0:
+            return None
+        if self._pool is None:
+            import torch
+            self._pool = [torch.cuda.Stream(device=f"cuda:{self.gpu}")
+                          for _ in range(self.nstreams)]
+        return self._pool
+
+    def take_slot(self):
+        self._slot += 1
+        return self._slot - 1
 
     def footprint(self, req, p=None):
         p = p or plan(req.shape)

@@ -378,9 +378,11 @@ def run_calls(pattern, *, Nc, Ntpb, Nts, Nas, Npolout, held=None, mfma=False, to
 
 
 def unsplit_products(Nts, Nas, Np=2, Nbps=8, mfma=MFMA_FAST):
-    """Products the matrix-core engine at level MFMA_FAST runs with unsplit
-    FP16 operands (1024-point, two pols, int8, >= 32 spectra per dump); they
-    are held to MFMA_BAR, every other product to the plain rules."""
-    if mfma != MFMA_FAST or Np != 2 or Nbps != 8:
+    """Products the engine at level MFMA_FAST runs below FP32 precision, held
+    to MFMA_BAR (every other product to the plain rules): 1024-point int8
+    products integrating >= 32 spectra (unsplit FP16 operands) and, with two
+    pols, four-step products (24-bit intermediate)."""
+    if mfma != MFMA_FAST or Np != 2:
         return {}
-    return {i: MFMA_BAR for i, (n, a) in enumerate(zip(Nts, Nas)) if n == 1024 and a >= 32}
+    return {i: MFMA_BAR for i, (n, a) in enumerate(zip(Nts, Nas))
+            if (n == 1024 and a >= 32 and Nbps == 8) or n >= FP32_FLOOR_NT}

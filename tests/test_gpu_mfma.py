@@ -164,3 +164,26 @@ def test_unsplit_with_other_products_and_switch():
     res, cfg, ev = run_case(mfma=MFMA_FAST, Nc=2, Ntpb=65536, Nts=[1024], Nas=[8], Npolout=[-4],
                             nbuf=2, tone=(0, 20.0, 300.0, 1024.0))
     check_results(res, cfg, tol=PLAIN_TOL)
+
+
+@pytest.mark.parametrize("nt", [1 << 17, 1 << 20])
+@pytest.mark.parametrize("mode", [1, 4, -4])
+def test_level1_fourstep_24bit_intermediate(nt, mode):
+    """Level 1 keeps four-step products' intermediate in 24-bit floats: one
+    spectrum per dump (the worst case: nothing averages) and an integration
+    over three buffers (held first-pass output), forward and inverse, with
+    the tone; within the 1e-3 bar of the engine against the FP64 oracle, and
+    within 1e-4 of the FP32 path on the same input (a numpy model of the
+    rounding gives at most 2e-5)."""
+    for nas, nbuf, fdir in [(1, 1, 1), (3, 3, -1)]:
+        kw = dict(Nc=2, Ntpb=nt, Nts=[nt], Nas=[nas], Npolout=[mode], nbuf=nbuf, fft_dir=fdir,
+                  conj=1, tone=(1, 20.0, 300.0, 1024.0))
+        print(f"case: level 1 {kw}", flush=True)
+        r1, cfg, _ = run_case(mfma=MFMA_FAST, **kw)
+        w = check_results(r1, cfg, tol_by_product={0: MFMA_BAR})
+        rv, _, _ = run_case(**kw)
+        d = max(rel_err(a[2], b[2], cfg.Nds[0], abs(mode), cfg.Nc, nt, mode).max()
+                for a, b in zip(r1, rv))
+        print(f"  vs oracle {w[0]:.2e}, vs the FP32 path {d:.2e}", flush=True)
+        assert d <= 1e-4
+        assert not all(np.array_equal(a[2], b[2]) for a, b in zip(r1, rv))   # 24-bit really ran

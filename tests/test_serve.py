@@ -175,10 +175,15 @@ def test_stream_results_same_alone_and_in_the_mix(batch, mfma):
     mix = S.Server(record_dumps=True, max_active=len(reqs), batch=batch, mfma=mfma).run(
         reqs, arrival_scale=0.0)
     assert mix["peak_active"] > 1
+    assert _check_alone_vs_mix(reqs, mix, batch=batch, mfma=mfma) >= len(reqs)
+
+
+def _check_alone_vs_mix(reqs, mix, **kw):
+    """Every stream's dumps in the mix are bit-identical to the stream run
+    alone (same server options kw, one slot)."""
     nd = 0
     for r in reqs:
-        alone = S.Server(record_dumps=True, max_active=1, batch=batch, mfma=mfma).run(
-            [r], arrival_scale=0.0)
+        alone = S.Server(record_dumps=True, max_active=1, **kw).run([r], arrival_scale=0.0)
         a, m = _by_key(alone["results"][r.sid]), _by_key(mix["results"][r.sid])
         p = S.plan(r.shape)
         expect = {(k - 1, i) for i in range(len(r.shape.Nts))
@@ -189,19 +194,14 @@ def test_stream_results_same_alone_and_in_the_mix(batch, mfma):
             if a[key][1] is not None:
                 assert np.array_equal(a[key][1].view(np.uint32), m[key][1].view(np.uint32))
             nd += 1
-    assert nd >= len(reqs)
+    return nd
 
 
-@pytest.mark.gpu
-@pytest.mark.parametrize("batch,mfma", [(1, False), (4, True)])
-def test_mixed_streams_match_the_oracle(batch, mfma):
+def _check_oracle(srv, reqs, mix, mfma):
+    """Every stream's dumps against the FP64 oracle fed the same buffers."""
     import torch
     from tests.spectro_cases import (MFMA_FLOOR_MULT, PLAIN_TOL, FP32_FLOOR_NT, Fp32Ref,
                                      rel_err)
-
-    reqs = _small_mix(n=6, seed=33)
-    srv = S.Server(record_dumps=True, max_active=len(reqs), batch=batch, mfma=mfma)
-    mix = srv.run(reqs, arrival_scale=0.0)
     for r in reqs:
         s, p = r.shape, S.plan(r.shape)
         ring = [b.cpu().numpy() for b in srv.make_ring(r, p)]   # same seed: same buffers
@@ -233,3 +233,42 @@ def test_mixed_streams_match_the_oracle(batch, mfma):
                                  cfg.Npolout[i]).max()
                     assert ei <= PLAIN_TOL, (r.sid, k, i, ei)
         st.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch,mfma", [(1, False), (4, True)])
+def test_mixed_streams_match_the_oracle(batch, mfma):
+    reqs = _small_mix(n=6, seed=33)
+    srv = S.Server(record_dumps=True, max_active=len(reqs), batch=batch, mfma=mfma)
+    mix = srv.run(reqs, arrival_scale=0.0)
+    _check_oracle(srv, reqs, mix, mfma)
+
+
+@pytest.mark.gpu
+def test_serving_shape_ranges_shared_streams():
+    """The serving workload's shape ranges at reduced channel counts: Nfft up
+    to 2^20, blocks of 16, 64 or 128 MiB, Nint up to 3072, 8 or 16 bits,
+    1-4 products, pols 1, 4 or -4, up to 4 channels (and antennas); this
+    seeded mix holds products of 2^20 (64 spectra), 65536, 8192, 1024 and
+    8 to 32 points, int8 and int16, 64 and 128 MiB buffers and a two-antenna
+    stream with the incoherent sum.  The mix runs its contexts on two shared
+    HIP streams; every stream's dumps are bit-identical to the stream run
+    alone on its own stream and match the FP64 oracle."""
+    def shape(rng):
+        # drawn from the ranges, redrawn until every product dumps within
+        # three buffers (so every product's output is checked)
+        while True:
+            sh = S.random_shape(rng, max_fft_log2=20, block_mib=(16, 64, 128), max_chans=4,
+                                max_nint=3072)
+            if max(S.plan(sh).Nis) <= 3:
+                return sh
+    reqs = S.make_requests(n=4, rate=1e4, seed=47, stream_bytes=384 * MiB,
+                           max_stream_bytes=512 * MiB, shape_fn=shape)
+    print("mix:", [(r.sid, r.shape.Nc, r.shape.Nbps, r.shape.Ntpb, r.shape.Nts, r.shape.Nas,
+                    r.shape.Npolout, r.shape.Nant, r.nbuf) for r in reqs], flush=True)
+    assert max(max(r.shape.Nts) for r in reqs) == 1 << 20
+    srv = S.Server(record_dumps=True, max_active=len(reqs), streams=2)
+    mix = srv.run(reqs, arrival_scale=0.0)
+    assert mix["peak_active"] > 1
+    assert _check_alone_vs_mix(reqs, mix) >= len(reqs)
+    _check_oracle(srv, reqs, mix, False)
