@@ -438,11 +438,15 @@ def main(argv=None):
                 for k, (n, ms) in prof.items()}
         if mfma:
             # the matrix-core engine at its 1e-3 bar runs the 1024-point
-            # product (3072 spectra per dump) on unsplit FP16 operands; the
-            # four-step passes and the 8-point product stay on the vector units
+            # product (3072 spectra per dump) on unsplit FP16 operands and
+            # keeps the 2^20 product's intermediate in 24-bit floats; both
+            # four-step passes and the 8-point product stay on the vector
+            # units, where they are faster
             names = {k.split(".")[1] for k in kern if k.endswith(".N1024")}
             assert names == {"k_small_mfma_u", "k_reduce"} or names == {"k_small_mfma_u"}, \
                 f"matrix-core leg ran {sorted(names)} for the 1024-point product"
+            assert "p0.k_fs_a.N1048576" in kern and "p0.k_fs_b.N1048576" in kern, \
+                f"matrix-core leg ran {sorted(kern)}"
         gbs = a.steps * buf_bytes / dt / 1e9
         log(f"bench: batch {batch} mfma {int(mfma)}: {a.steps} steps in {dt:.3f} s -> {gbs:.1f} GB/s "
             f"({dt / a.steps * 1e3:.3f} ms/step); " +
@@ -479,7 +483,7 @@ def main(argv=None):
     out_b, nis = per_prod[prod]                     # dump bytes of that product, buffers per dump
     if kname.startswith("k_small"):
         kb_step = buf_bytes + out_b / nis
-    elif kname == "k_fs_a":
+    elif kname.startswith("k_fs_a"):
         kb_step = buf_bytes
     else:
         kb_step = out_b / nis

@@ -98,14 +98,17 @@ void *rawspec_amd_stream(rawspec_context *ctx);
 int rawspec_amd_set_stream(rawspec_context *ctx, void *stream);
 
 /* Transform engine of the context.  enable = 0: FP32 on the vector units
- * (the default).  enable = 1: the matrix cores (FP16 MFMA with FP32
- * accumulation) wherever a matrix-core kernel exists for the product's shape,
- * the vector units elsewhere, held to the engine's 1e-3 relative error bar:
- * int8 products that integrate at least 32 spectra per dump use unsplit FP16
- * operands (about 2^-12 rounding per transform, random from spectrum to
- * spectrum: the error of an integration falls with its length, to a floor
- * near 1.5e-4), all others hi/lo split operands.  enable = 2: the matrix
- * cores with split operands everywhere (error at the FP32 level).  Other
+ * (the default; 1e-5 relative error bar).  enable = 1: the matrix-core engine
+ * at its 1e-3 relative error bar, using each kernel where it is faster: FP16
+ * matrix products with FP32 accumulation for 1024-point products (unsplit
+ * operands when an int8 product integrates at least 32 spectra per dump:
+ * about 2^-12 rounding per transform, random from spectrum to spectrum, so
+ * the error of an integration falls with its length to a floor near
+ * 1.5e-4; hi/lo split operands otherwise), the four-step intermediate in
+ * 24-bit floats (about 2e-5 added error at one spectrum per dump), and the
+ * vector units for everything else, the four-step passes included.
+ * enable = 2: every matrix-core kernel with split operands, the four-step
+ * first pass among them (error at the FP32 level).  Other
  * values are refused.  A product longer than 4096 points switches engine at
  * the start of its next integration, so a partial integration is never
  * mixed.  The getter returns the setting, -1 before initialisation. */

@@ -464,18 +464,28 @@ __device__ __forceinline__ unsigned int round24(float x) {
   const unsigned int b = __float_as_uint(x);
   return b + 0x7fu + ((b >> 8) & 1u);   // the upper three bytes hold the rounded value
 }
-__device__ __forceinline__ void store_y24(unsigned int *p, float4 v) {
+// One 12-byte access per value (global_{load,store}_dwordx3 at 4-byte
+// alignment): lanes holding consecutive values then cover a contiguous run
+// with each instruction.  Three dword accesses per value would leave 8-byte
+// gaps in every instruction's footprint, and non-temporal partial sectors
+// are written to HBM separately.
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3), aligned(4)));
+__device__ __forceinline__ u32x3 pack_y24(float4 v) {
   const unsigned int b0 = round24(v.x), b1 = round24(v.y), b2 = round24(v.z), b3 = round24(v.w);
-  __builtin_nontemporal_store(__builtin_amdgcn_perm(b1, b0, 0x05030201u), p);
-  __builtin_nontemporal_store(__builtin_amdgcn_perm(b2, b1, 0x06050302u), p + 1);
-  __builtin_nontemporal_store(__builtin_amdgcn_perm(b3, b2, 0x07060503u), p + 2);
+  u32x3 w;
+  w.x = __builtin_amdgcn_perm(b1, b0, 0x05030201u);
+  w.y = __builtin_amdgcn_perm(b2, b1, 0x06050302u);
+  w.z = __builtin_amdgcn_perm(b3, b2, 0x07060503u);
+  return w;
+}
+__device__ __forceinline__ void store_y24(unsigned int *p, float4 v) {
+  __builtin_nontemporal_store(pack_y24(v), reinterpret_cast<u32x3 *>(p));
 }
 __device__ __forceinline__ float4 load_y24(const unsigned int *p) {
-  const unsigned int w0 = __builtin_nontemporal_load(p), w1 = __builtin_nontemporal_load(p + 1),
-                     w2 = __builtin_nontemporal_load(p + 2);
-  return make_float4(__uint_as_float(w0 << 8), __uint_as_float(__builtin_amdgcn_perm(w1, w0, 0x0504030cu)),
-                     __uint_as_float(__builtin_amdgcn_perm(w2, w1, 0x0403020cu)),
-                     __uint_as_float(w2 & 0xffffff00u));
+  const u32x3 w = __builtin_nontemporal_load(reinterpret_cast<const u32x3 *>(p));
+  return make_float4(__uint_as_float(w.x << 8), __uint_as_float(__builtin_amdgcn_perm(w.y, w.x, 0x0504030cu)),
+                     __uint_as_float(__builtin_amdgcn_perm(w.z, w.y, 0x0403020cu)),
+                     __uint_as_float(w.z & 0xffffff00u));
 }
 
 // The running sum of an integration in its internal layout: each block is

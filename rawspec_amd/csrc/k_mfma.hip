@@ -441,6 +441,171 @@ static int go_b_mfma(const FsArgs &a, unsigned nb, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// k_fs_a_mfma: pass A of the four-step path on the matrix cores for
+// N2 = 1024 (N1 = R1 * 1024; launched for R1 = 1), two pols, int8 samples.
+// k_fs_a's contract and items (a tile of 4 consecutive n1 per workgroup and
+// spectrum, workgroups looping over the items); the tile (16 bytes per n2
+// row) is loaded cooperatively into LDS, the next item's into registers
+// meanwhile.  One wavefront transforms one n1 column: the 1024-point
+// DFT over n2 as split-FP16 matrix products (dft1024: int8 input exact, the
+// DFT matrix hi + lo in both stages, the stage-2 input hi + lo -- the error
+// of an FP32 transform), then the four-step twiddle W_N^(n1 k2) from the same
+// two LDS tables as k_fs_a, and Y in pass B's tiled layout [k2 / 4][n1][k2 %
+// 4], as two-pol FP32 values or 24-bit floats (Y24), staged through LDS so
+// that the workgroup's four columns leave as runs of 16 entries.
+template <int R1, bool Y24>
+__global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_fs_a_mfma(FsArgs a) {
+  constexpr int N2 = 1024, N1C = R1 * N2, L1 = ilog2(N1C), TA = kMfmaWaves, WG = 64 * kMfmaWaves;
+  constexpr int RPT = N2 / WG;         // tile rows each thread loads
+  static_assert(RPT == 4, "the tile prefetch holds four rows per thread in r0..r3");
+  using S = Sample<8, 2>;
+  __shared__ Dft1024K K;
+  __shared__ float2 twl[N2];           // W_N2^h, h < N2
+  __shared__ float2 twr[N1C];          // W_N^r, r < N1
+  // the item's samples: [n2][4 consecutive n1] words
+  __shared__ __attribute__((aligned(16))) unsigned int tin[N2 * TA];
+  // output staging of one quarter of an item: [wavefront][group][4 entries]
+  __shared__ __attribute__((aligned(16))) unsigned int st[kMfmaWaves * 64 * 4 * (Y24 ? 3 : 4)];
+  for (int m = threadIdx.x; m < N2; m += WG) twl[m] = a.tw[m * (kTwN / N2)];
+  for (int m = threadIdx.x; m < N1C; m += WG) twr[m] = a.twf[m];
+  dft1024_consts(K, a.tw, WG);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = lane & 31, h = lane >> 5;
+  const unsigned N1 = N1C;
+  const unsigned long long N = (unsigned long long)N1 * N2;
+  const unsigned tiles = N1 / TA;
+  const unsigned nsp = a.Nss * a.ynb;
+  const unsigned items = tiles * a.Nss * a.nbat * a.nch;
+
+  // the item's tile, 16 bytes (4 n1 of one n2 row) per thread and row:
+  // thread t loads rows t + WG q, q < RPT
+  auto tile_src = [&](unsigned l) -> const unsigned int * {
+    const unsigned tile = l % tiles;
+    const unsigned cs = l / tiles;
+    const unsigned sp = cs % a.Nss;
+    const unsigned cb = cs / a.Nss;
+    const unsigned b = cb % a.nbat;
+    const unsigned cl = cb / a.nbat;
+    return reinterpret_cast<const unsigned int *>(a.in[b] + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
+           (unsigned long long)sp * N + tile * TA + (unsigned long long)threadIdx.x * N1;
+  };
+  unsigned l = xcd_remap(blockIdx.x, gridDim.x);
+  uint4 r0, r1, r2, r3;
+  {
+    const unsigned int *p = tile_src(l);
+    r0 = *reinterpret_cast<const uint4 *>(p);
+    r1 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)WG * N1);
+    r2 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)2 * WG * N1);
+    r3 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)3 * WG * N1);
+  }
+  for (;;) {
+    // the tile into LDS (the previous item's readers are past the last
+    // barrier of its store loop)
+    reinterpret_cast<uint4 *>(tin)[threadIdx.x] = r0;
+    reinterpret_cast<uint4 *>(tin)[threadIdx.x + WG] = r1;
+    reinterpret_cast<uint4 *>(tin)[threadIdx.x + 2 * WG] = r2;
+    reinterpret_cast<uint4 *>(tin)[threadIdx.x + 3 * WG] = r3;
+    __syncthreads();                     // also orders the constants before first use
+    const unsigned ln = l + gridDim.x;   // uniform across the workgroup
+    if (ln < items) {
+      const unsigned int *p = tile_src(ln);
+      r0 = *reinterpret_cast<const uint4 *>(p);
+      r1 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)WG * N1);
+      r2 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)2 * WG * N1);
+      r3 = *reinterpret_cast<const uint4 *>(p + (unsigned long long)3 * WG * N1);
+    }
+    f16v xr[2], xi[2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      float2 x[4][4];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        float2 o[2];
+        S::cvt(tin[(col + 32 * (8 * (e >> 2) + 4 * h + (e & 3))) * TA + wv], o, a.isign);
+        x[e >> 2][e & 3] = o[p];
+      }
+      dft1024<false>(x, K, xr[p], xi[p]);   // X / 32
+    }
+    const unsigned tile = l % tiles;
+    const unsigned cs = l / tiles;
+    const unsigned sp = cs % a.Nss;
+    const unsigned cb = cs / a.Nss;
+    const unsigned b = cb % a.nbat;
+    const unsigned cl = cb / a.nbat;
+    const unsigned n1 = tile * TA + wv;
+    const unsigned long long spec = (unsigned long long)cl * nsp + (a.yb0 + b) * a.Nss + sp;
+    // Y entries ((k2 / 4) N1 + n1) * 4 + k2 % 4.  A lane holds k2 = 8 rq +
+    // 4 h + i + 32 col (i < 4) of its column for rq < 4; per rq the four
+    // wavefronts' values (4 consecutive n1) go through LDS so that 4 threads
+    // write one 16-entry run (k2 / 4 fixed, n1 = tile * 4 .. + 3, k2 % 4 =
+    // 0 .. 3): the run's 16-byte chunks c = w + 4 q go to thread w at store
+    // q, so each store instruction covers 64 contiguous bytes of every run
+    // (non-temporal stores with gaps reach HBM as separate partial writes).
+    constexpr int EW = Y24 ? 3 : 4;      // words per entry
+#pragma unroll
+    for (int rq = 0; rq < 4; ++rq) {
+      unsigned int *sw = st + ((wv * 64 + 2 * col + h) * 4) * EW;   // group g = 2 col + h
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * rq + i;
+        const unsigned k2 = 8 * rq + 4 * h + i + 32 * col;
+        const unsigned e = n1 * k2;      // < N <= 2^21
+        float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
+        w.x *= 32.0f;                    // undo the transform's 1/32
+        w.y *= 32.0f;
+        const float2 y0 = cmul(make_float2(xr[0][r], xi[0][r]), w);
+        const float2 y1 = cmul(make_float2(xr[1][r], xi[1][r]), w);
+        if constexpr (Y24) {
+          const unsigned int b0 = round24(y0.x), b1 = round24(y0.y), b2 = round24(y1.x),
+                             b3 = round24(y1.y);
+          sw[3 * i] = __builtin_amdgcn_perm(b1, b0, 0x05030201u);
+          sw[3 * i + 1] = __builtin_amdgcn_perm(b2, b1, 0x06050302u);
+          sw[3 * i + 2] = __builtin_amdgcn_perm(b3, b2, 0x07060503u);
+        } else {
+          reinterpret_cast<float4 *>(sw)[i] = make_float4(y0.x, y0.y, y1.x, y1.y);
+        }
+      }
+      __syncthreads();
+      {
+        const int t = threadIdx.x, g = t >> 2, w = t & 3;          // group, thread within the group
+        const unsigned k2q = 2 * rq + (g & 1) + 8 * (g >> 1);       // k2 / 4
+        // the run's first entry (n1 = tile * 4, k2 % 4 = 0); it is 4 slots
+        // (n1) of EW chunks
+        const unsigned long long ent = spec * N + ((unsigned long long)k2q * N1C + tile * TA) * 4;
+        unsigned int *dst = reinterpret_cast<unsigned int *>(a.work) + ent * EW;
+#pragma unroll
+        for (int q = 0; q < EW; ++q) {
+          const int c = w + 4 * q, slot = c / EW, part = c % EW;
+          const f32x4 v = *reinterpret_cast<const f32x4 *>(st + ((slot * 64 + g) * 4) * EW + 4 * part);
+          __builtin_nontemporal_store(v, reinterpret_cast<f32x4 *>(dst + 4 * c));
+        }
+      }
+      __syncthreads();
+    }
+    if (ln >= items) break;
+    l = ln;
+  }
+}
+
+// N1 = 1024 only: pass B's k2 tile is then 4, the run this kernel writes
+// (N1 = 2048 reads 2-wide tiles)
+int fs_a_mfma_ok(int n1, int n2, int nbps, int np) {
+  return n2 == 1024 && n1 == 1024 && nbps == 8 && np == 2;
+}
+
+int launch_fs_a_mfma(int nbps, int np, const FsArgs &a, hipStream_t s) {
+  if (!fs_a_mfma_ok((int)a.N1, (int)a.N2, nbps, np) || a.tby != (unsigned)kMfmaWaves ||
+      a.nbat == 0 || a.nbat > (unsigned)kMaxBatch || a.yb0 + a.nbat > a.ynb)
+    return 2;
+  const unsigned long long items = (unsigned long long)(a.N1 / kMfmaWaves) * a.Nss * a.nbat * a.nch;
+  const unsigned long long nb = a.grid_a && a.grid_a < items ? a.grid_a : items;
+  if (nb == 0 || items > 0x7fffffffull) return 2;
+  const dim3 g((unsigned)nb), t(64 * kMfmaWaves);
+  if (a.y24) hipLaunchKernelGGL((k_fs_a_mfma<1, true>), g, t, 0, s, a);
+  else hipLaunchKernelGGL((k_fs_a_mfma<1, false>), g, t, 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
 
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {

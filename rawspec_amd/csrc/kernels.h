@@ -116,6 +116,10 @@ int launch_small_mfma(int n, int nbps, int np, int pmode, const SmallArgs &a, un
 // floor near 1.5e-4, for products that integrate enough spectra.
 int launch_small_mfma_u(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
                         hipStream_t s);
+// Matrix-core pass A (N1 = N2 = 1024, int8, two pols): k_fs_a's contract;
+// a.grid_a workgroups loop over the items.
+int fs_a_mfma_ok(int n1, int n2, int nbps, int np);
+int launch_fs_a_mfma(int nbps, int np, const FsArgs &a, hipStream_t s);
 // Matrix-core pass B (N1 = 1024, two pols): a workgroup's 4 columns are one
 // k2 tile of Y (tby = fs_tile_b(1024) = 4).
 int fs_b_mfma_ok(int n1, int np);

@@ -114,12 +114,13 @@ const char *rsd_engine_name(void);
 int rsd_profile(rsd_ctx *d, int enable);
 int rsd_profile_report(rsd_ctx *d, char *buf, size_t len);
 
-/* Transform engine: 0 = FP32 vector units (default); 1 = matrix cores where
- * a kernel exists for the shape, at the engine's 1e-3 bar (unsplit FP16
- * operands for int8 products integrating at least 32 spectra, split FP16
- * elsewhere); 2 = matrix cores with split FP16 operands everywhere (FP32-level
- * error).  Other levels are refused (1).  A four-step product switches at
- * the start of its next integration. */
+/* Transform engine: 0 = FP32 vector units (default); 1 = the matrix-core
+ * engine at its 1e-3 bar (unsplit FP16 operands for int8 1024-point products
+ * integrating at least 32 spectra, split elsewhere, a 24-bit four-step
+ * intermediate, vector kernels where faster, both four-step passes included);
+ * 2 = matrix cores with split FP16 operands everywhere (FP32-level error).
+ * Other levels are refused (1).  A four-step product switches at the start
+ * of its next integration. */
 int rsd_set_mfma(rsd_ctx *d, int level);
 /* Two-lane execution on (default where it pays) or off. */
 int rsd_set_lanes(rsd_ctx *d, int enable);

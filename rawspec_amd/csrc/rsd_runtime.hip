@@ -64,6 +64,7 @@ struct ProdPlan {
   unsigned fs_bmax;        // buffers per batched launch pair (1: no batching)
   int fs_mfma;             // pass B on the matrix cores for the current integration
   int fs_y24;              // 24-bit intermediate for the current integration
+  int fs_amfma;            // pass A on the matrix cores for the current integration
   // launch-plan tunables (defaults below; rsd_autotune times alternatives)
   unsigned t_target;       // small kernel: workgroups a launch should reach
   unsigned t_maxspg;       // small kernel: most slots per dump group
@@ -720,6 +721,14 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
 // stays within its budget.
 // Arguments shared by every pass A/B launch of product i; latches the
 // integration's pass-B engine when it starts.
+// Lowest engine level that runs the four-step pass A on the matrix cores
+// where a kernel exists.  At level 1 the vector pass A is the faster one
+// (0.300 against 0.382 ms per 256 MiB buffer of the 2^20 product, both
+// storing 24-bit Y), so only level 2, the matrix cores everywhere, uses it.
+#ifndef RSD_FSA_MFMA_LEVEL
+#define RSD_FSA_MFMA_LEVEL 2
+#endif
+
 static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float gam, float scale,
                     int first) {
   const rsd_config &c = d->cfg;
@@ -753,6 +762,12 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   // it falls as the integration grows), fifty times under the 1e-3 bar.
   if (first) pp.fs_y24 = d->mfma == 1 && c.Np == 2 && !pp.fs_mfma;
   a.y24 = pp.fs_y24;
+  // Pass A on the matrix cores where a kernel exists (int8, two pols,
+  // N2 = 1024), from level RSD_FSA_MFMA_LEVEL up: split operands keep its
+  // error at the FP32 level.
+  if (first)
+    pp.fs_amfma = d->mfma != 0 && d->mfma >= RSD_FSA_MFMA_LEVEL &&
+                  rsk::fs_a_mfma_ok((int)pp.N1, (int)pp.N2, (int)c.Nbps, (int)c.Np);
   // both pass-B kernels read Y tiled by fs_tile_b(N1) = 4 consecutive k2
   // (for N1 = 1024, the matrix-core pass B's workgroup of 4 columns)
   a.tby = (unsigned)rsk::fs_tile_b((int)pp.N1);
@@ -764,6 +779,23 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   }
   a.yb0 = 0;
   a.grid_a = d->cus * RSD_FSA_WG_PER_CU;
+}
+
+// Matrix-core pass-A workgroups per CU (all resident: two per CU at its
+// register and LDS footprint), each looping over its share of the items.
+#ifndef RSD_FSA_MFMA_WG_PER_CU
+#define RSD_FSA_MFMA_WG_PER_CU 2
+#endif
+
+static int launch_pass_a(rsd_ctx *d, int i, rsk::FsArgs a, hipStream_t s) {
+  ProdPlan &pp = d->pp[i];
+  if (pp.fs_amfma) {
+    a.grid_a = d->cus * RSD_FSA_MFMA_WG_PER_CU;
+    ProfScope ps(d, i, "k_fs_a_mfma", pp.p.Nt, s);
+    return rsk::launch_fs_a_mfma((int)d->cfg.Nbps, (int)d->cfg.Np, a, s);
+  }
+  ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
+  return rsk::launch_fs_a((int)d->cfg.Nbps, (int)d->cfg.Np, a, s);
 }
 
 static int launch_pass_b(rsd_ctx *d, int i, const rsk::FsArgs &a, hipStream_t s) {
@@ -841,10 +873,7 @@ static int run_fourstep_held(rsd_ctx *d, int i, const uint8_t *in, float isign, 
   a.na_buf = pp.na_buf;
   a.Ng = pp.Ng;
   a.mode = rsk::FS_MID;      // pass A does not read it
-  {
-    ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
-    if (rsk::launch_fs_a((int)d->cfg.Nbps, (int)d->cfg.Np, a, s)) return 1;
-  }
+  if (launch_pass_a(d, i, a, s)) return 1;
   ++pp.held;
   if (!last && pp.held < pp.hold_use) return 0;
   return flush_held(d, i, a, last);
@@ -878,10 +907,7 @@ static int run_fourstep(rsd_ctx *d, int i, const uint8_t *const *ins, unsigned n
     for (unsigned c0 = 0; c0 < c.Nc; c0 += chc) {
       a.c0 = c0;
       a.nch = c.Nc - c0 < chc ? c.Nc - c0 : chc;
-      {
-        ProfScope ps(d, i, "k_fs_a", pp.p.Nt, s);
-        if (rsk::launch_fs_a((int)c.Nbps, (int)c.Np, a, s)) return 1;
-      }
+      if (launch_pass_a(d, i, a, s)) return 1;
       if (launch_pass_b(d, i, a, s)) return 1;
     }
     j += kb;

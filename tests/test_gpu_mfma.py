@@ -187,3 +187,23 @@ def test_level1_fourstep_24bit_intermediate(nt, mode):
         print(f"  vs oracle {w[0]:.2e}, vs the FP32 path {d:.2e}", flush=True)
         assert d <= 1e-4
         assert not all(np.array_equal(a[2], b[2]) for a, b in zip(r1, rv))   # 24-bit really ran
+
+
+@pytest.mark.parametrize("level", [MFMA_SPLIT, MFMA_FAST])
+@pytest.mark.parametrize("nt", [1 << 20, 1 << 21])
+def test_matrix_core_pass_a_selection(level, nt):
+    """Level 2 runs pass A on the matrix cores for 2^20 points (N1 = N2 =
+    1024) and on the vector units for 2^21 (N1 = 2048, whose pass B reads
+    2-wide k2 tiles); level 1 keeps the faster vector pass A (with the
+    24-bit intermediate).  Every case integrates over two buffers within its
+    level's bar."""
+    kw = dict(Nc=2, Ntpb=nt, Nts=[nt], Nas=[2], Npolout=[-4])
+    names = _kernels(level, **kw)
+    print(f"case: level {level} N {nt}: {sorted(names)}", flush=True)
+    on_mfma = level == MFMA_SPLIT and nt == 1 << 20
+    assert ("k_fs_a_mfma" in names) == on_mfma
+    assert ("k_fs_a" in names) == (not on_mfma)
+    res, cfg, _ = run_case(mfma=level, nbuf=2, tone=(1, 20.0, 300.0, 1024.0), **kw)
+    w = check_results(res, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT,
+                      tol_by_product=unsplit_products(kw["Nts"], kw["Nas"], mfma=level))
+    print(f"  worst {w[0]:.2e}", flush=True)
