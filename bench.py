@@ -584,7 +584,11 @@ def main(argv=None):
         "metric": METRIC, "value": round(value, 2), "unit": "GB/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "split-fp16 (MFMA, fp32 accumulate)" if head_name == "served_mfma" else "fp32",
+        # level 1: FP16 matrix products (FP32 accumulate) for the 1024-point
+        # product, FP32 vector transforms elsewhere, the 2^20 product's
+        # intermediate in 24-bit floats
+        "dtype": ("fp16-mfma (fp32 accumulate) + fp32; 24-bit four-step intermediate"
+                  if head_name == "served_mfma" else "fp32"),
         "data": "synthetic",
         "config": {"workload": "stokes_64ch_int8_3products: 64 ch x 2 pol complex int8 (sigma 8, tone ch3 bin 300/1024), "
                                "blocks of 524288 samples, Nb=2 (256 MiB/step), "
