@@ -2,8 +2,15 @@
 ended by an END card (padded with blanks to a multiple of 512 bytes when
 DIRECTIO = 1), then BLOCSIZE bytes of samples [chan][time][pol][re, im];
 repeated per block.  Files of a stem are STEM.0000.raw, STEM.0001.raw, ...
-Reading them is the C library's job (rawspec_raw_read_header)."""
+Reading them is the C library's job (rawspec_raw_read_header).
+
+As a program it writes a synthetic observation, e.g. 32 blocks of 64
+channels x 2 pols of complex int8 Gaussian noise, 524288 samples per block:
+    python -m rawspec_amd.rawfile STEM --nchan 64 --ntpb 524288 --blocks 32"""
 from __future__ import annotations
+
+import argparse
+import sys
 
 
 def card(key: str, value) -> str:
@@ -34,3 +41,47 @@ def observation_cards(*, obsnchan, npol, nbits, ntpb, pktidx, obsfreq=1420.0, ob
             ("DIRECTIO", directio), ("NANTS", nants), ("SRC_NAME", src_name),
             ("RA_STR", ra_str), ("DEC_STR", dec_str), ("STT_IMJD", stt_imjd),
             ("STT_SMJD", stt_smjd), *extra]
+
+
+def write_noise(stem: str, *, nchan: int, npol: int = 2, nbits: int = 8, ntpb: int,
+                nblocks: int, sigma: float = 8.0, seed: int = 1, directio: int = 1) -> str:
+    """STEM.0000.raw with nblocks blocks of complex Gaussian noise of `sigma`
+    counts per component, rounded and clamped to the SNORM range of nbits
+    (8 or 16); PKTIDX advances by 32 per block.  Returns the file name."""
+    import numpy as np
+    if nbits not in (8, 16):
+        raise ValueError("nbits must be 8 or 16")
+    lim, dt = (127, np.int8) if nbits == 8 else (32767, np.int16)
+    rng = np.random.default_rng(seed)
+    name = stem + ".0000.raw"
+    with open(name, "wb") as f:
+        for b in range(nblocks):
+            f.write(header(observation_cards(obsnchan=nchan, npol=npol, nbits=nbits, ntpb=ntpb,
+                                             pktidx=32 * b, directio=directio), directio))
+            x = rng.standard_normal(nchan * ntpb * npol * 2, dtype=np.float32)
+            x *= np.float32(sigma)
+            np.rint(x, out=x)
+            np.clip(x, -lim, lim, out=x)
+            f.write(x.astype(dt).tobytes())
+    return name
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="write a GUPPI RAW file of complex Gaussian noise")
+    ap.add_argument("stem", help="writes STEM.0000.raw")
+    ap.add_argument("--nchan", type=int, required=True)
+    ap.add_argument("--npol", type=int, default=2)
+    ap.add_argument("--nbits", type=int, default=8)
+    ap.add_argument("--ntpb", type=int, required=True, help="time samples per block")
+    ap.add_argument("--blocks", type=int, default=1)
+    ap.add_argument("--sigma", type=float, default=8.0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--directio", type=int, default=1)
+    a = ap.parse_args(argv)
+    print(write_noise(a.stem, nchan=a.nchan, npol=a.npol, nbits=a.nbits, ntpb=a.ntpb,
+                      nblocks=a.blocks, sigma=a.sigma, seed=a.seed, directio=a.directio))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

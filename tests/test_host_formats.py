@@ -474,3 +474,31 @@ def test_telescope_ids_match_reference(L, ref):
     _, ref_fb = ref
     for t in TELESCOPES:
         assert L.fb_telescope_id(t.encode()) == ref_fb.fb_telescope_id(t.encode()), t
+
+
+@pytest.mark.parametrize("nbits,directio", [(8, 1), (16, 0)])
+def test_noise_file_reads_back(L, nbits, directio):
+    """rawfile.write_noise: every block's header parses with the block
+    geometry asked for, blocks follow each other at header + BLOCSIZE, the
+    samples are the clamped rounded noise (mean near 0, sigma near the one
+    asked for, integral, in range), and the file ends after the last block."""
+    import numpy as np
+    from rawspec_amd.rawfile import write_noise
+    nchan, npol, ntpb, nblocks, sigma = 3, 2, 256, 3, 8.0
+    with tempfile.TemporaryDirectory() as d:
+        name = write_noise(os.path.join(d, "s"), nchan=nchan, npol=npol, nbits=nbits, ntpb=ntpb,
+                           nblocks=nblocks, sigma=sigma, seed=5, directio=directio)
+        blocs = nchan * ntpb * npol * 2 * nbits // 8
+        walk = _read_all(L, name)
+        assert [p > 0 for p, _f in walk] == [True] * nblocks + [False]
+        ends = []
+        for b, (pos, f) in enumerate(walk[:-1]):
+            assert (f["obsnchan"], f["npol"], f["nbits"], f["blocsize"], f["pktidx"],
+                    f["directio"]) == (nchan, npol, nbits, blocs, 32 * b, directio)
+            if directio:
+                assert pos % 512 == 0
+            ends.append(pos + blocs)
+        assert ends[-1] == os.path.getsize(name)
+        raw = open(name, "rb").read()
+        x = np.frombuffer(raw[walk[0][0]:walk[0][0] + blocs], dtype=np.int8 if nbits == 8 else np.int16)
+        assert abs(x.mean()) < 0.5 and abs(x.std() - sigma) < 0.5
