@@ -80,7 +80,8 @@ static void usage(const char *argv0)
           "      --slots=N          Pinned staging buffers for pipelined ingest [2]\n"
           "      --sync-ingest      Read, copy and start each buffer in turn (the library's\n"
           "                         standard copy API) instead of the pipelined ingest\n"
-          "      --mfma             Use the matrix-core transform engine\n"
+          "      --mfma             Use the matrix-core transform engine at its 1e-3\n"
+          "                         relative error bar (engine level 1)\n"
           "      --stats            Print ingest statistics as one JSON line on stderr\n"
           "\n"
           "  -h, --help             Show this message\n"
