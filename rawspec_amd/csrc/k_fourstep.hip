@@ -190,7 +190,9 @@ void k_fs_b(FsArgs a) {
   const unsigned tiles = N2 / TB;
   const unsigned l = xcd_remap(blockIdx.x, gridDim.x);
   const unsigned tile = l % tiles;
-  const unsigned cg = l / tiles;
+  const unsigned rg = l / tiles;
+  const unsigned run = rg % a.ns;
+  const unsigned cg = rg / a.ns;
   const unsigned g = cg % a.Ng;
   const unsigned cl = cg / a.Ng;
   const unsigned c = a.c0 + cl;
@@ -205,7 +207,9 @@ void k_fs_b(FsArgs a) {
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[m][q] = 0.0f;
 
-  for (unsigned sp = g * a.na_buf; sp < (g + 1) * a.na_buf; ++sp) {
+  const unsigned sp0 = g * a.na_buf + run * a.sp_run;
+  const unsigned sp1 = min(sp0 + a.sp_run, (g + 1) * a.na_buf);
+  for (unsigned sp = sp0; sp < sp1; ++sp) {
     // this workgroup's tile: N1 rows of TB k2 values, contiguous
     const unsigned long long e0 = ((unsigned long long)cl * a.Nss * a.ynb + sp) * N +
                                   (unsigned long long)tile * N1 * TB;   // first value of the tile
@@ -233,6 +237,17 @@ void k_fs_b(FsArgs a) {
   }
 #pragma unroll
   for (int m = 0; m < E; ++m) detect_finish<PMODE>(acc[m], a.gam);
+
+  if (a.ns > 1) {
+    // this run's sums, unscaled, in the internal accumulator's block layout
+    float *pb = a.partb + ((((unsigned long long)run * a.Ng + g) * a.nch + cl) * tiles + tile) *
+                              Q * N1 * TB + (unsigned long long)j * TB + sg;
+#pragma unroll
+    for (int m = 0; m < E; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) store_acc(pb + ((unsigned long long)q * N1 + m * T) * TB, acc[m][q]);
+    return;
+  }
 
   // internal accumulator block of this workgroup: [Q][N1][TB]; thread
   // element (q, m) at ib + (q*N1 + m*T)*TB.  The mode is uniform, so each
@@ -342,11 +357,18 @@ static int go_b(int np, int pmode, const FsArgs &a, unsigned nb, hipStream_t s) 
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// The runs must cover every spectrum of a group, and a split launch needs
+// its partial-sum buffer.
+static bool runs_ok(const FsArgs &a) {
+  return a.ns >= 1 && (unsigned long long)a.ns * a.sp_run >= a.na_buf &&
+         (a.ns == 1 || a.partb != nullptr);
+}
+
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
   if (a.y24 && np != 2) return 2;
   const unsigned tiles = a.N2 / fs_tile_b((int)a.N1);
-  const unsigned long long nb = (unsigned long long)tiles * a.Ng * a.nch;
-  if (nb == 0 || nb > 0x7fffffffull || a.nbat == 0 || a.nbat > a.ynb) return 2;
+  const unsigned long long nb = (unsigned long long)tiles * a.ns * a.Ng * a.nch;
+  if (nb == 0 || nb > 0x7fffffffull || a.nbat == 0 || a.nbat > a.ynb || !runs_ok(a)) return 2;
   switch (a.N1) {
     case 128: return go_b<128>(np, pmode, a, (unsigned)nb, s);
     case 256: return go_b<256>(np, pmode, a, (unsigned)nb, s);
@@ -356,6 +378,55 @@ int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s) {
     case 4096: return go_b<4096>(np, pmode, a, (unsigned)nb, s);
   }
   return 2;
+}
+
+// One thread per value of the launch's sums: the ns partial sums in run
+// order, then the finish k_fs_b applies to its own sums (same scale, same
+// internal-accumulator block, same shifted output position).
+__global__ __launch_bounds__(256) void k_fs_b_reduce(FsArgs a, unsigned Q, unsigned TB) {
+  const unsigned N1 = a.N1, N2 = a.N2, tiles = N2 / TB;
+  const unsigned long long N = (unsigned long long)N1 * N2;
+  const unsigned long long total = (unsigned long long)a.Ng * a.nch * tiles * Q * N1 * TB;
+  for (unsigned long long idx = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < total; idx += (unsigned long long)gridDim.x * blockDim.x) {
+    // idx runs over [Ng][nch][tiles][Q][N1][TB], one run's block of partb
+    const unsigned sg = (unsigned)(idx % TB);
+    unsigned long long r = idx / TB;
+    const unsigned k1 = (unsigned)(r % N1);
+    r /= N1;
+    const unsigned q = (unsigned)(r % Q);
+    r /= Q;
+    const unsigned tile = (unsigned)(r % tiles);
+    r /= tiles;
+    const unsigned cl = (unsigned)(r % a.nch);
+    const unsigned g = (unsigned)(r / a.nch);
+    float sum = 0.0f;
+    for (unsigned run = 0; run < a.ns; ++run) sum += load_acc(a.partb + run * total + idx);
+    float v = sum * a.scale;
+    const unsigned c = a.c0 + cl;
+    float *ie = a.acc_int + ((((unsigned long long)g * a.Nc + c) * tiles + tile) * Q + q) * N1 * TB +
+                (unsigned long long)k1 * TB + sg;
+    if (a.mode == FS_MID || a.mode == FS_LAST) v += load_acc(ie);
+    if (a.mode == FS_FIRST || a.mode == FS_MID) {
+      store_acc(ie, v);
+    } else {
+      const unsigned k2 = tile * TB + sg;
+      a.acc[((unsigned long long)g * Q * a.Nc + c) * N + k2 + (unsigned long long)q * a.Nc * N +
+            (unsigned long long)N2 * ((k1 + N1 / 2) & (N1 - 1))] = v;
+    }
+  }
+}
+
+int launch_fs_b_reduce(int pmode, const FsArgs &a, hipStream_t s) {
+  if (a.ns < 2 || !runs_ok(a)) return 2;
+  const unsigned Q = pmode == 1 ? 1u : 4u;
+  const unsigned TB = (unsigned)fs_tile_b((int)a.N1);
+  const unsigned long long total = (unsigned long long)a.Ng * a.nch * (a.N2 / TB) * Q * a.N1 * TB;
+  if (total == 0) return 2;
+  unsigned long long nb = (total + 255) / 256;
+  if (nb > 16384) nb = 16384;
+  hipLaunchKernelGGL(k_fs_b_reduce, dim3((unsigned)nb), dim3(256), 0, s, a, Q, TB);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 }  // namespace rsk

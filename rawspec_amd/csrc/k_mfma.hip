@@ -609,8 +609,9 @@ int launch_fs_a_mfma(int nbps, int np, const FsArgs &a, hipStream_t s) {
 int fs_b_mfma_ok(int n1, int np) { return n1 == 1024 && np == 2; }
 
 int launch_fs_b_mfma(int np, int pmode, const FsArgs &a, hipStream_t s) {
+  // one workgroup per (k2 tile, group, channel) integrates the whole group
   if (!fs_b_mfma_ok((int)a.N1, np) || a.y24 || a.N2 % kMfmaWaves != 0 || !(a.mscale > 0.0f) ||
-      a.nbat == 0 || a.nbat > a.ynb || a.tby != (unsigned)kMfmaWaves)
+      a.nbat == 0 || a.nbat > a.ynb || a.tby != (unsigned)kMfmaWaves || a.ns != 1)
     return 2;
   const unsigned long long nb = (unsigned long long)(a.N2 / kMfmaWaves) * a.Ng * a.nch;
   if (nb == 0 || nb > 0x7fffffffull) return 2;

@@ -77,6 +77,12 @@ struct FsArgs {
   unsigned tby;                   // k2 tile of Y: fs_tile_b(N1), both pass-B kernels
   float mscale;                   // k_fs_b_mfma: power-of-two input scale (fs_mfma_scale)
   int y24;                        // two pols: Y holds 12-byte (24-bit float) values
+  // vector pass B: each dump group's na_buf spectra run in ns runs of at
+  // most sp_run spectra, one workgroup per (run, k2 tile, group, channel);
+  // with ns > 1 the runs' sums go to partb, laid out [ns][Ng][nch][tiles]
+  // [Q][N1][tby], and k_fs_b_reduce adds them and finishes the launch
+  unsigned ns, sp_run;
+  float *partb;
 };
 
 struct ReduceArgs {
@@ -106,6 +112,9 @@ int launch_small(int n, int nbps, int np, int pmode, const SmallArgs &a,
                  unsigned nblocks, hipStream_t s);
 int launch_fs_a(int nbps, int np, const FsArgs &a, hipStream_t s);
 int launch_fs_b(int np, int pmode, const FsArgs &a, hipStream_t s);
+// The ns runs' sums of a split pass B, in run order, then pass B's own
+// finish (scale, internal accumulator, shifted output) for a.mode.
+int launch_fs_b_reduce(int pmode, const FsArgs &a, hipStream_t s);
 // Matrix-core fused small kernel (N = 1024, two pols): one wavefront per
 // (channel, group, chunk) item, items = Nc * Ng * G, gpw must be 1.
 int small_mfma_ok(int n, int np);

@@ -9,6 +9,8 @@
 //           dump is one contiguous device->host copy
 //   d_ics   per product, [Nds][|P|][Nc/Nant][Nt] float
 //   d_part  partial integrations of groups split over workgroups
+//   d_partb partial sums of a four-step pass B whose dump groups are split
+//           into runs of spectra (products with few k2 tiles and channels)
 //   d_work  four-step intermediate of one channel chunk (several buffers'
 //           spectra when batched), tiled for pass B
 //   d_held  per four-step product integrating over several buffers, the
@@ -111,6 +113,14 @@ constexpr size_t kLaneMinBytes = (size_t)64 << 20;
 #define RSD_HOLD_MB 8192
 #endif
 constexpr size_t kHoldBytes = (size_t)RSD_HOLD_MB << 20;
+// Vector pass B: workgroups per CU a launch aims for, splitting each dump
+// group's spectra into runs when its (k2 tile, group, channel) workgroups
+// are fewer; the fewest spectra a run integrates; the CU count the planner
+// sizes the runs' partial sums for (launches use the device's own count,
+// capped by what was allocated).
+constexpr unsigned kFsbWgPerCu = 4;
+constexpr unsigned kFsbMinRun = 8;
+constexpr unsigned kPlanCus = 256;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
@@ -151,6 +161,8 @@ struct rsd_ctx {
   size_t part_len;
   float2 *d_work;
   size_t work_len;
+  float *d_partb;
+  size_t partb_len;
   unsigned cus;                        // compute units of the device
   unsigned hold_max;                   // buffers a product may hold (rsd_set_hold)
   size_t held_bytes;                   // allocated for held output
@@ -167,6 +179,8 @@ struct rsd_ctx {
   size_t sv_part_len;
   float2 *sv_work;
   size_t sv_work_len;
+  float *sv_partb;
+  size_t sv_partb_len;
   size_t sv_budget[RSD_MAX_PRODUCTS];
   bool prof;
   struct Rec { std::string name; hipEvent_t a, b; };
@@ -369,6 +383,7 @@ void rsd_destroy(rsd_ctx *d) {
   if (d->d_stage) (void)hipFree(d->d_stage);
   if (d->d_part) (void)hipFree(d->d_part);
   if (d->d_work) (void)hipFree(d->d_work);
+  if (d->d_partb) (void)hipFree(d->d_partb);
   if (d->d_tw) (void)hipFree(d->d_tw);
   if (d->d_w) (void)hipFree(d->d_w);
   for (auto &r : d->recs) {
@@ -397,6 +412,51 @@ static unsigned fs_chunk_for(const rsd_ctx *d, const ProdPlan &pp, unsigned kb) 
   const size_t per_chan = (size_t)kb * pp.p.Nss * pp.p.Nt * d->cfg.Np * sizeof(float2);
   const size_t ch = pp.fs_budget / per_chan;
   return (unsigned)(ch < 1 ? 1 : (ch > d->cfg.Nc ? d->cfg.Nc : ch));
+}
+
+// Runs each dump group of a vector pass-B launch with na_buf spectra per
+// group is split into, for cus compute units: enough for about kFsbWgPerCu
+// workgroups per CU where the (k2 tile, group, channel) workgroups of all
+// Nc channels are fewer, each run at least kFsbMinRun spectra.  A workgroup
+// otherwise integrates its whole group in turn: a 2^13-point product over 8
+// channels is 128 workgroups, each summing thousands of spectra.  The count
+// depends on the context's channels, not on a launch's channel chunk, so
+// every path over the same buffers (batched, held, allocation-free with
+// its smaller chunks) adds the same runs in the same order.
+static unsigned fsb_runs(const ProdPlan &pp, unsigned cus, unsigned Nc, unsigned na_buf) {
+  const unsigned long long base =
+      (unsigned long long)(pp.N2 / (unsigned)rsk::fs_tile_b((int)pp.N1)) * pp.Ng * Nc;
+  const unsigned long long target = (unsigned long long)cus * kFsbWgPerCu;
+  if (base == 0 || base >= target) return 1;
+  unsigned long long ns = (target + base - 1) / base;
+  const unsigned maxr = na_buf / kFsbMinRun;
+  if (ns > maxr) ns = maxr;
+  return ns < 1 ? 1u : (unsigned)ns;
+}
+
+// floats of partial sums a pass-B launch split into ns runs writes
+static size_t fsb_part_floats(const ProdPlan &pp, unsigned ns, unsigned nch) {
+  return ns < 2 ? 0 : (size_t)ns * pp.Ng * nch * pp.Q * pp.p.Nt;
+}
+
+// The most partial sums product pp's pass-B launches need under its plan:
+// every channel chunk size the batched and one-buffer paths use, and the
+// held path over all channels.
+static size_t fsb_part_need(const rsd_ctx *d, const ProdPlan &pp) {
+  if (pp.small) return 0;
+  size_t need = 0;
+  auto take = [&](unsigned nch, unsigned na) {
+    if (nch == 0) return;
+    const size_t n = fsb_part_floats(pp, fsb_runs(pp, kPlanCus, d->cfg.Nc, na), nch);
+    if (n > need) need = n;
+  };
+  for (unsigned kb = 1; kb <= pp.fs_bmax; ++kb) {
+    const unsigned chc = fs_chunk_for(d, pp, kb);
+    take(chc, kb * pp.na_buf);
+    take(d->cfg.Nc % chc, kb * pp.na_buf);
+  }
+  if (pp.hold_k >= 2) take(d->cfg.Nc, pp.hold_k * pp.na_buf);
+  return need;
 }
 
 // Launch plan of one product from its tunables; returns the float2
@@ -456,7 +516,7 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
   d->chan_block_bytes = (unsigned long long)cfg->Ntpb * d->sample_bytes;
   d->chan_stride = d->chan_block_bytes * cfg->Nb;
   d->in_bytes = (size_t)(d->chan_stride * cfg->Nc);
-  size_t part_max = 0, work_max = 0;
+  size_t part_max = 0, work_max = 0, partb_max = 0;
   for (unsigned i = 0; i < cfg->No; ++i) {
     ProdPlan &pp = d->pp[i];
     pp.p = cfg->prod[i];
@@ -478,11 +538,14 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
     if (wl > work_max) work_max = wl;
     const size_t pn = part_need(d, pp);
     if (pn > part_max) part_max = pn;
+    const size_t pb = fsb_part_need(d, pp);
+    if (pb > partb_max) partb_max = pb;
     pp.acc_len = (size_t)pp.Q * pp.p.Nds * cfg->Nc * N;
     pp.ics_len = cfg->ics ? pp.acc_len / cfg->Nant : 0;
   }
   d->part_len = part_max;
   d->work_len = work_max;
+  d->partb_len = partb_max;
   return 0;
 }
 
@@ -503,7 +566,8 @@ static size_t plan_bytes(const rsd_ctx *d) {
       if (pp.hold_k >= 2 && held + h <= kHoldBytes) held += h;
     }
   }
-  return b + held + d->part_len * sizeof(float) + d->work_len * sizeof(float2);
+  return b + held + d->part_len * sizeof(float) + d->work_len * sizeof(float2) +
+         d->partb_len * sizeof(float);
 }
 
 static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
@@ -544,6 +608,7 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
   }
   if (alloc_zero((void **)&d->d_part, d->part_len * sizeof(float))) return 1;
   if (alloc_zero((void **)&d->d_work, d->work_len * sizeof(float2))) return 1;
+  if (d->partb_len && alloc_zero((void **)&d->d_partb, d->partb_len * sizeof(float))) return 1;
   RSD_CHECK(hipStreamCreateWithFlags(&d->own_stream, hipStreamNonBlocking));
   d->stream = d->own_stream;
   RSD_CHECK(hipEventCreateWithFlags(&d->ev_main, hipEventDisableTiming));
@@ -570,6 +635,8 @@ static rsd_ctx *new_ctx() {
   d->d_in = d->d_stage = nullptr;
   d->d_part = nullptr;
   d->d_work = d->d_tw = nullptr;
+  d->d_partb = nullptr;
+  d->partb_len = 0;
   d->hold_max = rsk::kMaxBatch;
   d->cus = 1;
   d->held_bytes = 0;
@@ -779,6 +846,9 @@ static void fs_args(rsd_ctx *d, ProdPlan &pp, rsk::FsArgs &a, float isign, float
   }
   a.yb0 = 0;
   a.grid_a = d->cus * RSD_FSA_WG_PER_CU;
+  a.ns = 1;                  // pass B's runs are chosen per launch
+  a.sp_run = 0;
+  a.partb = nullptr;
 }
 
 // Matrix-core pass-A workgroups per CU (all resident: two per CU at its
@@ -798,14 +868,32 @@ static int launch_pass_a(rsd_ctx *d, int i, rsk::FsArgs a, hipStream_t s) {
   return rsk::launch_fs_a((int)d->cfg.Nbps, (int)d->cfg.Np, a, s);
 }
 
-static int launch_pass_b(rsd_ctx *d, int i, const rsk::FsArgs &a, hipStream_t s) {
+static int launch_pass_b(rsd_ctx *d, int i, rsk::FsArgs a, hipStream_t s) {
   ProdPlan &pp = d->pp[i];
+  a.ns = 1;
+  a.sp_run = a.na_buf;
+  a.partb = nullptr;
   if (pp.fs_mfma) {
     ProfScope ps(d, i, "k_fs_b_mfma", pp.p.Nt, s);
     return rsk::launch_fs_b_mfma((int)d->cfg.Np, pp.p.npolout, a, s);
   }
-  ProfScope ps(d, i, "k_fs_b", pp.p.Nt, s);
-  return rsk::launch_fs_b((int)d->cfg.Np, pp.p.npolout, a, s);
+  // split the dump groups into runs where the launch is too narrow for the
+  // GPU, as far as the partial-sum buffer allows
+  unsigned ns = fsb_runs(pp, d->cus < kPlanCus ? d->cus : kPlanCus, d->cfg.Nc, a.na_buf);
+  const size_t per_run = fsb_part_floats(pp, 2, a.nch) / 2;
+  if (ns > 1 && per_run && (size_t)ns * per_run > d->partb_len) ns = (unsigned)(d->partb_len / per_run);
+  if (ns > 1) {
+    a.sp_run = (a.na_buf + ns - 1) / ns;
+    a.ns = (a.na_buf + a.sp_run - 1) / a.sp_run;   // no empty run
+    a.partb = d->d_partb;
+  }
+  {
+    ProfScope ps(d, i, "k_fs_b", pp.p.Nt, s);
+    if (int rc = rsk::launch_fs_b((int)d->cfg.Np, pp.p.npolout, a, s)) return rc;
+  }
+  if (a.ns < 2) return 0;
+  ProfScope ps(d, i, "k_fs_b_reduce", pp.p.Nt, s);
+  return rsk::launch_fs_b_reduce(pp.p.npolout, a, s);
 }
 
 // Pass B over the held buffers of product i, which complete the integration
@@ -1021,21 +1109,24 @@ int rsd_noalloc_begin(rsd_ctx *d) {
     fflush(stderr);
     return 1;
   }
-  // the small products' partial sums first, then the four-step intermediate
+  // the small products' partial sums, pass B's partial sums (the same
+  // capacity as the context's own, so pass B splits its groups the same
+  // way), then the four-step intermediate
   size_t part = 0;
   for (unsigned i = 0; i < d->cfg.No; ++i) {
     const size_t pn = part_need(d, d->pp[i]);
     if (pn > part) part = pn;
   }
   const size_t part_bytes = (part * sizeof(float) + 255) / 256 * 256;
-  if (part_bytes > kScratchBytes) {
+  const size_t partb_bytes = (d->partb_len * sizeof(float) + 255) / 256 * 256;
+  if (part_bytes + partb_bytes > kScratchBytes) {
     fprintf(stderr,
             "librawspec: allocation-free processing needs %zu MiB of partial sums; the scratch "
-            "holds %zu MiB\n", (part_bytes + (1u << 20) - 1) >> 20, kScratchBytes >> 20);
+            "holds %zu MiB\n", (part_bytes + partb_bytes + (1u << 20) - 1) >> 20, kScratchBytes >> 20);
     fflush(stderr);
     return 1;
   }
-  const size_t work_bytes = kScratchBytes - part_bytes;
+  const size_t work_bytes = kScratchBytes - part_bytes - partb_bytes;
   for (unsigned i = 0; i < d->cfg.No; ++i) {
     const ProdPlan &pp = d->pp[i];
     if (pp.small) continue;
@@ -1044,7 +1135,7 @@ int rsd_noalloc_begin(rsd_ctx *d) {
       fprintf(stderr,
               "librawspec: allocation-free processing needs %zu MiB of intermediate for one "
               "channel of product %u; the scratch holds %zu MiB beside %zu MiB of partial sums\n",
-              per_chan >> 20, i, kScratchBytes >> 20, part_bytes >> 20);
+              per_chan >> 20, i, kScratchBytes >> 20, (part_bytes + partb_bytes) >> 20);
       fflush(stderr);
       return 1;
     }
@@ -1053,9 +1144,15 @@ int rsd_noalloc_begin(rsd_ctx *d) {
   d->sv_part_len = d->part_len;
   d->sv_work = d->d_work;
   d->sv_work_len = d->work_len;
+  d->sv_partb = d->d_partb;
+  d->sv_partb_len = d->partb_len;
+  d->d_partb = d->partb_len ? reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(base) +
+                                                        part_bytes)
+                            : nullptr;
   d->d_part = reinterpret_cast<float *>(base);
   d->part_len = part;
-  d->d_work = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(base) + part_bytes);
+  d->d_work = reinterpret_cast<float2 *>(reinterpret_cast<unsigned char *>(base) + part_bytes +
+                                         partb_bytes);
   d->work_len = work_bytes / sizeof(float2);
   for (unsigned i = 0; i < d->cfg.No; ++i) {
     ProdPlan &pp = d->pp[i];
@@ -1072,6 +1169,8 @@ void rsd_noalloc_end(rsd_ctx *d) {
   d->part_len = d->sv_part_len;
   d->d_work = d->sv_work;
   d->work_len = d->sv_work_len;
+  d->d_partb = d->sv_partb;
+  d->partb_len = d->sv_partb_len;
   for (unsigned i = 0; i < d->cfg.No; ++i) d->pp[i].fs_budget = d->sv_budget[i];
   d->noalloc = false;
 }

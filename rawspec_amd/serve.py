@@ -19,9 +19,14 @@ coexist in one process).  The loop does continuous batching, not queueing:
     budgets for the next arrival.
 
 Block-to-dump latency is measured per dump: from the call that submitted the
-buffer completing an integration to the POST dump callback for that dump
-(queueing in front of the GPU included, admission wait excluded and reported
-separately).
+buffer completing an integration to the moment the loop sees that call
+complete (its event, polled every sweep; queueing in front of the GPU
+included, admission wait excluded and reported separately).  Contexts get no
+dump callback then: each callback is a host function that holds the
+context's stream, and the hardware queue it shares with other contexts,
+until it returns, and enqueueing one costs about 25 us of host time per
+call.  With record_dumps the POST dump callback copies each dump (the host
+buffers are reused by the next dump) and times it instead.
 
 There is no reference counterpart: the reference serves one stream per
 process (its texture object and callback pointers are process globals).
@@ -184,7 +189,8 @@ class _Stream:
         self.lat = []
         self.dumps = [] if srv.record_dumps else None
         self.sp = Spectrometer(**req.shape.spectrometer_kwargs(), gpu_index=srv.gpu,
-                               callback=self._on_dump, record_events=False)
+                               callback=self._on_dump if srv.record_dumps else None,
+                               record_events=False)
         if self.sp.set_device_output(not srv.record_dumps):
             raise RuntimeError("rawspec_amd_set_device_output failed")
         if self.sp.set_mfma(srv.mfma):
@@ -220,8 +226,15 @@ class _Stream:
         """Retires completed buffers and queues new ones; True if it did either."""
         import torch
         moved = False
-        while self.inflight and self.inflight[0].query():
-            self.inflight.popleft()
+        while self.inflight and self.inflight[0][0].query():
+            _ev, s0, nb = self.inflight.popleft()
+            if self.dumps is None:
+                # dumps completed by buffers s0 + 1 .. s0 + nb (1-based)
+                now = time.perf_counter()
+                for p, nis in enumerate(self.plan.Nis):
+                    for k in range((s0 // nis + 1) * nis, s0 + nb + 1, nis):
+                        self.ndump[p] += 1
+                        self.lat.append(now - self.t_sub[k - 1])
             moved = True
         while len(self.inflight) < self.depth and self.sent < self.req.nbuf:
             nb = min(self.batch, self.req.nbuf - self.sent)
@@ -234,7 +247,7 @@ class _Stream:
                 raise RuntimeError(f"stream {self.req.sid}: start processing failed")
             ev = torch.cuda.Event()
             ev.record(self.hs)
-            self.inflight.append(ev)
+            self.inflight.append((ev, self.sent, nb))
             self.sent += nb
             moved = True
         return moved

@@ -178,6 +178,23 @@ def test_stream_results_same_alone_and_in_the_mix(batch, mfma):
     assert _check_alone_vs_mix(reqs, mix, batch=batch, mfma=mfma) >= len(reqs)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch", [1, 3])
+def test_event_timed_latency_counts_every_dump(batch):
+    """Without recorded dumps the contexts get no callback and each dump is
+    timed from the completion event of the call that completes it: one
+    latency per dump a product completes (nbuf // Nis each), as many as the
+    callback-timed run of the same requests counts, all positive and no
+    longer than that stream's whole run."""
+    reqs = _small_mix(n=6, seed=23)
+    want = sum(r.nbuf // nis for r in reqs for nis in S.plan(r.shape).Nis)
+    print("case: event-timed dumps, batch", batch, "expected", want, flush=True)
+    ev = S.Server(max_active=len(reqs), batch=batch).run(reqs, arrival_scale=0.0)
+    cb = S.Server(record_dumps=True, max_active=len(reqs), batch=batch).run(reqs, arrival_scale=0.0)
+    assert ev["dumps"] == cb["dumps"] == want > 0
+    assert 0.0 < ev["p50_ms"] <= ev["p99_ms"] <= ev["wall_s"] * 1e3
+
+
 def _check_alone_vs_mix(reqs, mix, **kw):
     """Every stream's dumps in the mix are bit-identical to the stream run
     alone (same server options kw, one slot)."""

@@ -82,6 +82,11 @@ def test_noalloc_matches_device_call():
     # smaller: fewer channels, shorter blocks, other products
     _case(R, torch, Nc=2, Ntpb=32768, Nts=[16384, 512], Nas=[4, 64],
           Npolout=[4, 1], nbuf=4, seed=12)
+    # four-step products whose pass B splits each dump group into runs of
+    # spectra (partial sums in the scratch): dumps within a buffer (8192
+    # points, 2 groups of 16 spectra) and across four buffers (16384 points)
+    _case(R, torch, Nc=2, Ntpb=1 << 18, Nts=[8192, 16384, 8], Nas=[16, 64, 128],
+          Npolout=[-4, 4, 1], nbuf=4, seed=13)
 
 
 def test_noalloc_refuses_oversized_scratch():
