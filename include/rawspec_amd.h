@@ -75,7 +75,9 @@ int rawspec_amd_start_processing_device_batch(rawspec_context *ctx, const void *
                                               unsigned int nbuf, int fft_dir);
 
 /* enable != 0: dumps stay in device memory (no copy into h_pwrbuf/h_icsbuf);
- * the PRE/POST callbacks still fire in stream order.  The dumped spectra of
+ * the PRE/POST callbacks still fire in stream order, both from one host
+ * function per dump (no copy separates them; every host function holds the
+ * stream until it returns).  The dumped spectra of
  * product i (same layout as h_pwrbuf[i]) are then valid at
  * rawspec_amd_device_pwrbuf(ctx, i) from POST(i) until the next call that
  * starts processing on this context. */
