@@ -542,17 +542,27 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_fs_a_mfma(FsArgs a) {
     // q, so each store instruction covers 64 contiguous bytes of every run
     // (non-temporal stores with gaps reach HBM as separate partial writes).
     constexpr int EW = Y24 ? 3 : 4;      // words per entry
+    // Four-step twiddle W_N^(n1 k2), k2 = (8 rq + 4 h + i) + 32 col, as
+    // W_N^(n1 (8 rq + 4 h + i)) (two distinct table entries per instruction:
+    // a broadcast) times this lane's W_N^(32 n1 col), looked up once per
+    // item.  Looking up W_N^(n1 k2) directly puts the 32 lanes of a
+    // half-wave 32 n1 entries apart in the tables: one LDS bank, a 32-way
+    // conflict per lookup.
+    float2 wc;
+    {
+      const unsigned e2 = 32u * n1 * (unsigned)col;   // < 2^20 * R1
+      wc = cmul(twl[e2 >> L1], twr[e2 & (N1C - 1)]);
+      wc.x *= 32.0f;                     // undo the transform's 1/32
+      wc.y *= 32.0f;
+    }
 #pragma unroll
     for (int rq = 0; rq < 4; ++rq) {
       unsigned int *sw = st + ((wv * 64 + 2 * col + h) * 4) * EW;   // group g = 2 col + h
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = 4 * rq + i;
-        const unsigned k2 = 8 * rq + 4 * h + i + 32 * col;
-        const unsigned e = n1 * k2;      // < N <= 2^21
-        float2 w = cmul(twl[e >> L1], twr[e & (N1C - 1)]);
-        w.x *= 32.0f;                    // undo the transform's 1/32
-        w.y *= 32.0f;
+        const unsigned e1 = n1 * (unsigned)(8 * rq + 4 * h + i);  // < 32 N1
+        const float2 w = cmul(cmul(twl[e1 >> L1], twr[e1 & (N1C - 1)]), wc);
         const float2 y0 = cmul(make_float2(xr[0][r], xi[0][r]), w);
         const float2 y1 = cmul(make_float2(xr[1][r], xi[1][r]), w);
         if constexpr (Y24) {

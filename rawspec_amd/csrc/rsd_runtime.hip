@@ -118,8 +118,11 @@ constexpr size_t kHoldBytes = (size_t)RSD_HOLD_MB << 20;
 // are fewer; the fewest spectra a run integrates; the CU count the planner
 // sizes the runs' partial sums for (launches use the device's own count,
 // capped by what was allocated).
+#ifndef RSD_FSB_MIN_RUN
+#define RSD_FSB_MIN_RUN 8
+#endif
 constexpr unsigned kFsbWgPerCu = 4;
-constexpr unsigned kFsbMinRun = 8;
+constexpr unsigned kFsbMinRun = RSD_FSB_MIN_RUN;
 constexpr unsigned kPlanCus = 256;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
@@ -790,7 +793,7 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
 // integration's pass-B engine when it starts.
 // Lowest engine level that runs the four-step pass A on the matrix cores
 // where a kernel exists.  At level 1 the vector pass A is the faster one
-// (0.300 against 0.382 ms per 256 MiB buffer of the 2^20 product, both
+// (0.300 against 0.360 ms per 256 MiB buffer of the 2^20 product, both
 // storing 24-bit Y), so only level 2, the matrix cores everywhere, uses it.
 #ifndef RSD_FSA_MFMA_LEVEL
 #define RSD_FSA_MFMA_LEVEL 2
