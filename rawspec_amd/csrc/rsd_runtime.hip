@@ -442,24 +442,17 @@ static size_t fsb_part_floats(const ProdPlan &pp, unsigned ns, unsigned nch) {
   return ns < 2 ? 0 : (size_t)ns * pp.Ng * nch * pp.Q * pp.p.Nt;
 }
 
-// The most partial sums product pp's pass-B launches need under its plan:
-// every channel chunk size the batched and one-buffer paths use, and the
-// held path over all channels.
+// The most partial sums product pp's pass-B launches can need: a launch
+// over all Nc channels with the most spectra per group any path can give
+// it (a batch or held group of min(kMaxBatch, Nis) buffers; the run count
+// grows with the spectra).  Launches over a channel chunk or fewer buffers
+// need less, so whatever the chunking and batching (autotuned budgets, the
+// allocation-free scratch) the run count is never cut by the buffer.
 static size_t fsb_part_need(const rsd_ctx *d, const ProdPlan &pp) {
   if (pp.small) return 0;
-  size_t need = 0;
-  auto take = [&](unsigned nch, unsigned na) {
-    if (nch == 0) return;
-    const size_t n = fsb_part_floats(pp, fsb_runs(pp, kPlanCus, d->cfg.Nc, na), nch);
-    if (n > need) need = n;
-  };
-  for (unsigned kb = 1; kb <= pp.fs_bmax; ++kb) {
-    const unsigned chc = fs_chunk_for(d, pp, kb);
-    take(chc, kb * pp.na_buf);
-    take(d->cfg.Nc % chc, kb * pp.na_buf);
-  }
-  if (pp.hold_k >= 2) take(d->cfg.Nc, pp.hold_k * pp.na_buf);
-  return need;
+  const unsigned kb = pp.p.Nis < (unsigned)rsk::kMaxBatch ? pp.p.Nis : (unsigned)rsk::kMaxBatch;
+  const unsigned na = (kb > 1 ? kb : 1) * pp.na_buf;
+  return fsb_part_floats(pp, fsb_runs(pp, kPlanCus, d->cfg.Nc, na), d->cfg.Nc);
 }
 
 // Launch plan of one product from its tunables; returns the float2
