@@ -34,8 +34,17 @@ constexpr int small_lds_f2() {
   constexpr int RED_F2 = SLOTS >= 2 ? ((SLOTS / 2) * Q * N + 1) / 2 : 1;
   return FFT_F2 > RED_F2 ? FFT_F2 : RED_F2;
 }
-// at most 256 registers per thread, so two waves share each SIMD
-template <int N, int NBPS, int NP, int PMODE>
+// at most 256 registers per thread, so two waves share each SIMD.
+//
+// BS: a transform length NT = a.nt that is not a power of two, by
+// Bluestein's identity with the padded length N >= 2 NT - 1 (a power of
+// two):  X[k] = c[k] * sum_n (x[n] c[n]) conj(c[k - n]),  c[n] =
+// exp(-i pi n^2 / NT), the sum a cyclic convolution of length N.  The
+// samples times c (zero past NT) go through the team FFT, times a.bsk =
+// FFT_N(the conjugate chirp, wrapped) / N, then through the team FFT again
+// conjugated (an inverse FFT), and conjugated times c; bins k >= NT are
+// zero and dropped.
+template <int N, int NBPS, int NP, int PMODE, bool BS = false>
 __global__ __launch_bounds__(wg_for(N)) __attribute__((amdgpu_waves_per_eu(2)))
 void k_small(SmallArgs a) {
   constexpr int E = elems_for(N);
@@ -68,6 +77,8 @@ void k_small(SmallArgs a) {
 
   const float2 *twp = tw_setup<N, WG>(twl, a.tw);
   const W *src = reinterpret_cast<const W *>(a.in + (unsigned long long)c * a.chan_stride);
+  // transform length (samples per spectrum, bins per output spectrum)
+  const unsigned NT = BS ? a.nt : (unsigned)N;
   const unsigned s0 = (gvalid ? g : 0u) * a.na_buf + chunk * a.chunk;
 
   float acc[E][Q];
@@ -80,10 +91,17 @@ void k_small(SmallArgs a) {
   for (int it = 0; it < iters; ++it) {
     const unsigned sl = (unsigned)(si + it * a.spg);
     const bool valid = gvalid && sl < a.chunk;
-    const unsigned long long sbase = (unsigned long long)(s0 + (valid ? sl : 0u)) * N;
+    const unsigned long long sbase = (unsigned long long)(s0 + (valid ? sl : 0u)) * NT;
     float2 v[NP][E];
     W raw[E];
-    if constexpr (T == 1 && (E * sizeof(W)) % 16 == 0) {
+    if constexpr (BS) {
+      // samples past NT are the padding (and may lie past the buffer)
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const unsigned n = (unsigned)(j + m * T);
+        raw[m] = n < NT ? src[sbase + n] : W{};
+      }
+    } else if constexpr (T == 1 && (E * sizeof(W)) % 16 == 0) {
       // one thread per transform: the spectrum is contiguous, 16-byte loads
       const uint4 *s4 = reinterpret_cast<const uint4 *>(src + sbase);
       uint4 *r4 = reinterpret_cast<uint4 *>(raw);
@@ -95,7 +113,7 @@ void k_small(SmallArgs a) {
     }
     // one thread per int8 transform: samples offset by 128, removed from bin
     // 0 (the thread's element 0) after the single-stage transform
-    constexpr bool kOffset = T == 1 && NBPS == 8;
+    constexpr bool kOffset = T == 1 && NBPS == 8 && !BS;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       float2 o[NP];
@@ -104,8 +122,36 @@ void k_small(SmallArgs a) {
 #pragma unroll
       for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
+    if constexpr (BS) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const unsigned n = (unsigned)(j + m * T);
+        const float2 cn = n < NT ? a.chirp[n] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) v[p][m] = cmul(v[p][m], cn);
+      }
+    }
     // a team is T consecutive threads: within one wavefront when T <= 64
     team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp);
+    if constexpr (BS) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const float2 bk = a.bsk[j + m * T];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          const float2 z = cmul(v[p][m], bk);
+          v[p][m] = make_float2(z.x, -z.y);
+        }
+      }
+      team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const unsigned k = (unsigned)(j + m * T);
+        const float2 ck = k < NT ? a.chirp[k] : make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) v[p][m] = cmul(make_float2(v[p][m].x, -v[p][m].y), ck);
+      }
+    }
     if constexpr (kOffset) {
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
@@ -133,22 +179,29 @@ void k_small(SmallArgs a) {
     // (consecutive lanes hold consecutive bins).
     const unsigned gg = g0 + slot;
     if (gg >= a.Ng) return;
-    float *bg = a.direct ? base + (((unsigned long long)gg * Q) * a.Nc + c) * N
-                         : base + ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q) * N;
-    const unsigned long long qstride = a.direct ? (unsigned long long)a.Nc * N : N;
+    float *bg = a.direct ? base + (((unsigned long long)gg * Q) * a.Nc + c) * NT
+                         : base + ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q) * NT;
+    const unsigned long long qstride = a.direct ? (unsigned long long)a.Nc * NT : NT;
     const float sc = a.direct ? a.scale : 1.0f;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       float *bq = bg + q * qstride;
       float val[E];
+      unsigned pos[E];                    // output position of bin j + m*T
 #pragma unroll
-      for (int m = 0; m < E; ++m) val[m] = acc[m][q] * sc;
+      for (int m = 0; m < E; ++m) {
+        val[m] = acc[m][q] * sc;
+        const unsigned k = (unsigned)(j + m * T);
+        pos[m] = BS ? (k + NT / 2) % NT : (k + N / 2) & (N - 1);
+      }
       if (accum) {
 #pragma unroll
-        for (int m = 0; m < E; ++m) val[m] += bq[(j + m * T + N / 2) & (N - 1)];
+        for (int m = 0; m < E; ++m)
+          if (!BS || (unsigned)(j + m * T) < NT) val[m] += bq[pos[m]];
       }
 #pragma unroll
-      for (int m = 0; m < E; ++m) bq[(j + m * T + N / 2) & (N - 1)] = val[m];   // bin j + m*T
+      for (int m = 0; m < E; ++m)
+        if (!BS || (unsigned)(j + m * T) < NT) bq[pos[m]] = val[m];
     }
     return;
   }
@@ -180,18 +233,19 @@ void k_small(SmallArgs a) {
   // accumulating) the batch's previous values are loaded together before the
   // stores, so read-modify-writes do not serialise on memory latency.
   constexpr int OB = 4;
-  const int nout = a.gpw * Q * N;
+  const int nout = a.gpw * Q * (int)NT;
   for (int i0 = tid; i0 < nout; i0 += OB * WG) {
     float val[OB];
     unsigned long long dst[OB];
 #pragma unroll
     for (int u = 0; u < OB; ++u) {
       const int idx = i0 + u * WG;
-      const int go = idx / (Q * N);
-      const int r2 = idx - go * (Q * N);
-      const int q = r2 / N;
-      const int jo = r2 - q * N;                   // output (shifted) fine channel
-      const int k = (jo + N / 2) & (N - 1);        // FFT bin stored there
+      const int go = idx / (Q * (int)NT);
+      const int r2 = idx - go * (Q * (int)NT);
+      const int q = r2 / (int)NT;
+      const int jo = r2 - q * (int)NT;             // output (shifted) fine channel
+      // FFT bin stored there: (k + NT/2) mod NT = jo
+      const int k = BS ? (int)((jo + NT - NT / 2) % NT) : (jo + N / 2) & (N - 1);
       const unsigned gg = g0 + go;
       dst[u] = ~0ull;
       val[u] = 0.0f;
@@ -200,10 +254,10 @@ void k_small(SmallArgs a) {
         float sum = rp[0];
         for (int s2 = 1; s2 < half; ++s2) sum += rp[s2 * Q * N];
         if (a.direct) {
-          dst[u] = (((unsigned long long)gg * Q + q) * a.Nc + c) * N + jo;
+          dst[u] = (((unsigned long long)gg * Q + q) * a.Nc + c) * NT + jo;
           val[u] = sum * a.scale;
         } else {
-          dst[u] = ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q + q) * N + jo;
+          dst[u] = ((((unsigned long long)c * a.Ng + gg) * a.G + chunk) * Q + q) * NT + jo;
           val[u] = sum;
         }
       }
@@ -224,6 +278,12 @@ int small_slots(int n) { return small_wg(n) / (n / elems_for(n) > 0 ? n / elems_
 
 template <int N, int NBPS, int NP, int PMODE>
 static int go(const SmallArgs &a, unsigned nb, hipStream_t s) {
+  if constexpr (N >= 8) {
+    if (a.nt != (unsigned)N) {
+      hipLaunchKernelGGL((k_small<N, NBPS, NP, PMODE, true>), dim3(nb), dim3(wg_for(N)), 0, s, a);
+      return hipGetLastError() == hipSuccess ? 0 : 1;
+    }
+  }
   hipLaunchKernelGGL((k_small<N, NBPS, NP, PMODE>), dim3(nb), dim3(wg_for(N)), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
@@ -262,6 +322,10 @@ static int go_n(int n, int np, int pmode, const SmallArgs &a, unsigned nb, hipSt
 int launch_small(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned nblocks,
                  hipStream_t s) {
   if (nblocks == 0) return 0;
+  // a Bluestein product: 2 <= nt, 2 nt - 1 <= n, with both tables
+  if (a.nt != (unsigned)n &&
+      (n < 8 || a.nt < 2 || 2ull * a.nt - 1 > (unsigned)n || !a.chirp || !a.bsk))
+    return 2;
   if (nbps == 8) return go_n<8>(n, np, pmode, a, nblocks, s);
   if (nbps == 16) return go_n<16>(n, np, pmode, a, nblocks, s);
   return 2;

@@ -33,6 +33,11 @@ struct SmallArgs {
   float *acc;                     // [Nds][Q][Nc][N], FFT-shifted
   float *part;                    // [Nc][Ng][G][Q][N] unscaled, FFT-shifted
   const float2 *tw;               // W_4096^m, m < 4096
+  // transform length: the kernel's N, or for a product whose length is not
+  // a power of two (Bluestein, N >= 2 nt - 1) nt with its tables
+  unsigned nt;
+  const float2 *chirp;            // c[n] = exp(-i pi n^2 / nt), n < nt
+  const float2 *bsk;              // FFT_N of the wrapped conjugate chirp, over N
 };
 
 // Four-step path (N = N1*N2 > 4096), launched per chunk of channels

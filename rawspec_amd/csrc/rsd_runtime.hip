@@ -52,7 +52,14 @@ namespace {
 struct ProdPlan {
   rsd_product p;
   int Q;
-  int small;               // N <= 4096: fused single kernel
+  int small;               // N <= 4096 (or Bluestein): fused single kernel
+  // a length that is not a power of two (<= kBsMaxN): Bluestein's identity
+  // in the small kernel at the padded power-of-two length bs_m >= 2 N - 1,
+  // with the chirp c[n] = exp(-i pi n^2 / N) and the padded kernel's FFT
+  int bs;
+  unsigned bs_m;
+  float2 *d_chirp;
+  float2 *d_bsk;
   // small path
   unsigned na_buf, Ng, chunk, G, gpw, spg, bpc;
   int direct;
@@ -127,6 +134,62 @@ constexpr unsigned kPlanCus = 256;
 
 bool is_pow2(unsigned n) { return n && !(n & (n - 1)); }
 unsigned ilog2u(unsigned n) { unsigned l = 0; while (n > 1) { n >>= 1; ++l; } return l; }
+
+// Longest transform that is not a power of two: Bluestein's identity pads
+// it to at least 2 N - 1 points, and the small kernel holds 4096.
+constexpr unsigned kBsMaxN = 2048;
+
+// Bluestein tables of an N-point transform padded to M points, computed in
+// FP64 and rounded once: chirp[n] = exp(-i pi n^2 / N) (n^2 reduced mod 2N
+// exactly), and bsk = FFT_M(b) / M with b[m] = conj(chirp[|m|]) for
+// |m| < N (indices mod M), zero elsewhere.
+static void bluestein_tables(unsigned N, unsigned M, std::vector<float2> &chirp,
+                             std::vector<float2> &bsk) {
+  std::vector<double> cr(N), ci(N);
+  for (unsigned n = 0; n < N; ++n) {
+    const unsigned long long r = ((unsigned long long)n * n) % (2ull * N);
+    const double ang = -M_PI * (double)r / (double)N;
+    cr[n] = cos(ang);
+    ci[n] = sin(ang);
+  }
+  std::vector<double> br(M, 0.0), bi(M, 0.0);
+  for (unsigned m = 0; m < N; ++m) {
+    br[m] = cr[m];
+    bi[m] = -ci[m];
+    if (m) {
+      br[M - m] = cr[m];
+      bi[M - m] = -ci[m];
+    }
+  }
+  // iterative radix-2 FFT, exp(-2 pi i k m / M)
+  for (unsigned i = 1, j = 0; i < M; ++i) {
+    unsigned bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(br[i], br[j]);
+      std::swap(bi[i], bi[j]);
+    }
+  }
+  for (unsigned len = 2; len <= M; len <<= 1) {
+    for (unsigned k = 0; k < len / 2; ++k) {
+      const double ang = -2.0 * M_PI * (double)k / (double)len;
+      const double wr = cos(ang), wi = sin(ang);
+      for (unsigned i = 0; i < M; i += len) {
+        const unsigned u = i + k, v = i + k + len / 2;
+        const double tr = br[v] * wr - bi[v] * wi, ti = br[v] * wi + bi[v] * wr;
+        br[v] = br[u] - tr;
+        bi[v] = bi[u] - ti;
+        br[u] += tr;
+        bi[u] += ti;
+      }
+    }
+  }
+  chirp.resize(N);
+  bsk.resize(M);
+  for (unsigned n = 0; n < N; ++n) chirp[n] = make_float2((float)cr[n], (float)ci[n]);
+  for (unsigned m = 0; m < M; ++m) bsk[m] = make_float2((float)(br[m] / M), (float)(bi[m] / M));
+}
 
 // forward twiddles exp(-2 pi i (m * mul) / n) for m < cnt, FP64 -> FP32
 std::vector<float2> twiddles(unsigned long long n, unsigned long long cnt, unsigned long long mul) {
@@ -235,8 +298,7 @@ struct ProfScope {
 };
 
 static void plan_small(rsd_ctx *d, ProdPlan &pp) {
-  const unsigned N = pp.p.Nt;
-  const unsigned slots = (unsigned)rsk::small_slots((int)N);
+  const unsigned slots = (unsigned)rsk::small_slots((int)(pp.bs ? pp.bs_m : pp.p.Nt));
   const unsigned Nc = d->cfg.Nc;
   pp.na_buf = pp.p.Na < pp.p.Nss ? pp.p.Na : pp.p.Nss;
   pp.Ng = pp.p.Nss / pp.na_buf;
@@ -379,6 +441,8 @@ void rsd_destroy(rsd_ctx *d) {
     if (pp.d_acc) (void)hipFree(pp.d_acc);
     if (pp.d_ics) (void)hipFree(pp.d_ics);
     if (pp.d_twf) (void)hipFree(pp.d_twf);
+    if (pp.d_chirp) (void)hipFree(pp.d_chirp);
+    if (pp.d_bsk) (void)hipFree(pp.d_bsk);
     if (pp.d_acc_int) (void)hipFree(pp.d_acc_int);
     if (pp.d_held) (void)hipFree(pp.d_held);
   }
@@ -517,15 +581,21 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
     ProdPlan &pp = d->pp[i];
     pp.p = cfg->prod[i];
     const unsigned N = pp.p.Nt;
-    if (!is_pow2(N) || N > (1u << 24)) {
+    if (N == 0 || N > (1u << 24) || (!is_pow2(N) && N > kBsMaxN)) {
       fprintf(stderr,
-              "librawspec: Nts[%u] = %u is not supported (powers of two up to 2^24 only)\n", i,
-              N);
+              "librawspec: Nts[%u] = %u is not supported (powers of two up to 2^24, other "
+              "lengths up to %u)\n", i, N, kBsMaxN);
       fflush(stderr);
       return 1;
     }
     pp.Q = pp.p.npolout == 1 ? 1 : 4;
-    pp.small = N <= (unsigned)rsk::kTwN;
+    pp.bs = !is_pow2(N);
+    pp.bs_m = N;
+    if (pp.bs) {
+      pp.bs_m = 1;
+      while (pp.bs_m < 2 * N - 1) pp.bs_m <<= 1;
+    }
+    pp.small = pp.bs || N <= (unsigned)rsk::kTwN;
     pp.t_target = 2048;
     pp.t_maxspg = 16;
     pp.t_mtarget = 2048;
@@ -553,6 +623,7 @@ static size_t plan_bytes(const rsd_ctx *d) {
   for (unsigned i = 0; i < d->cfg.No; ++i) {
     const ProdPlan &pp = d->pp[i];
     b += pp.acc_len * sizeof(float) + pp.ics_len * sizeof(float);
+    if (pp.bs) b += ((size_t)pp.p.Nt + pp.bs_m) * sizeof(float2);
     if (!pp.small) {
       b += (size_t)pp.N1 * sizeof(float2);
       if (pp.p.Nis > 1) b += pp.acc_len * sizeof(float);
@@ -598,6 +669,14 @@ static int create_impl(const rsd_config *cfg, rsd_ctx *d) {
       RSD_CHECK(hipMalloc(&pp.d_twf, f.size() * sizeof(float2)));
       RSD_CHECK(hipMemcpy(pp.d_twf, f.data(), f.size() * sizeof(float2), hipMemcpyHostToDevice));
       if (pp.p.Nis > 1 && alloc_zero((void **)&pp.d_acc_int, pp.acc_len * sizeof(float))) return 1;
+    }
+    if (pp.bs) {
+      std::vector<float2> ch, bk;
+      bluestein_tables(N, pp.bs_m, ch, bk);
+      RSD_CHECK(hipMalloc(&pp.d_chirp, ch.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_chirp, ch.data(), ch.size() * sizeof(float2), hipMemcpyHostToDevice));
+      RSD_CHECK(hipMalloc(&pp.d_bsk, bk.size() * sizeof(float2)));
+      RSD_CHECK(hipMemcpy(pp.d_bsk, bk.data(), bk.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
     if (alloc_zero((void **)&pp.d_acc, pp.acc_len * sizeof(float))) return 1;
     if (cfg->ics && alloc_zero((void **)&pp.d_ics, pp.ics_len * sizeof(float))) return 1;
@@ -728,6 +807,9 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
     a.acc = pp.d_acc;
     a.part = d->d_part;
     a.tw = d->d_tw;
+    a.nt = pp.p.Nt;
+    a.chirp = pp.d_chirp;
+    a.bsk = pp.d_bsk;
     // Both engines write the same (host) layout, so a product can change
     // engine between any two buffers.
     const bool mf = d->mfma && pp.m_ok;
@@ -753,7 +835,7 @@ int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first,
       const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
       if (nb > 0x7fffffffull) return 1;
       ProfScope ps(d, i, "k_small", pp.p.Nt, s);
-      if (rsk::launch_small((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
+      if (rsk::launch_small((int)pp.bs_m, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
                             s))
         return 1;
     }

@@ -148,8 +148,9 @@ def test_plan_validates_like_initialize_and_sizes_buffers(rlib, capfd):
             c.Nts[i], c.Nas[i], c.Npolout[i] = kw["Nts"][i], kw["Nas"][i], kw["Npolout"][i]
         dev, host = C.c_size_t(0), C.c_size_t(0)
         rc = rlib.lib().rawspec_amd_plan(C.byref(c), C.byref(dev), C.byref(host))
-        pow2 = all(n & (n - 1) == 0 for n in kw["Nts"][:kw["No"]])
-        if orc != 0 or not pow2:
+        # lengths: powers of two, or any other length up to 2048 (Bluestein)
+        lengths_ok = all(n & (n - 1) == 0 or n <= 2048 for n in kw["Nts"][:kw["No"]])
+        if orc != 0 or not lengths_ok:
             ninvalid += 1
             assert rc == 1, kw
             continue
