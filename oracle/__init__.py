@@ -16,6 +16,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# the same code built in single precision: the FP32 CPU port the CPU
+# baseline times beside the FP64 oracle; never the checker
+_LIB32_PATH = os.path.join(_HERE, "liboracle32.so")
 MAX_OUTPUTS = 4
 
 # Rule codes returned by ro_derive (the reference returns 1 for all).
@@ -28,7 +31,7 @@ ERRORS = {
 
 def build(force: bool = False) -> str:
     """Compile liboracle.so with the committed Makefile."""
-    if force or not os.path.exists(_LIB_PATH):
+    if force or not (os.path.exists(_LIB_PATH) and os.path.exists(_LIB32_PATH)):
         subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
                        check=True)
     return _LIB_PATH
@@ -56,14 +59,14 @@ class Cfg(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
+def lib(fp32: bool = False):
+    """The FP64 oracle library, or with fp32 the single-precision port."""
+    if fp32 not in _libs:
         build()
-        L = C.CDLL(_LIB_PATH)
+        L = C.CDLL(_LIB32_PATH if fp32 else _LIB_PATH)
         dp = C.POINTER(C.c_double)
         L.ro_derive.argtypes = [C.POINTER(Cfg)]
         L.ro_derive.restype = C.c_int
@@ -91,8 +94,8 @@ def lib():
                                    C.c_uint, C.c_uint64, C.c_double, C.c_int,
                                    C.c_ulong, C.c_int, C.c_double, C.c_double,
                                    C.c_double]
-        _lib = L
-    return _lib
+        _libs[fp32] = L
+    return _libs[fp32]
 
 
 def make_cfg(No=1, Np=2, Nant=1, Nc=1, Ntpb=1024, Nbps=8, Nbc=0, Nb=0,
@@ -172,27 +175,29 @@ def device_layout(blocks, Nc, Ntpb, Np) -> np.ndarray:
 
 
 class State:
-    """Streaming FP64 oracle for one configuration."""
+    """Streaming FP64 oracle for one configuration (fp32: the single-
+    precision port of the same code, for CPU-baseline timing only)."""
 
-    def __init__(self, cfg: Cfg, weights=None, threads: int = 0):
+    def __init__(self, cfg: Cfg, weights=None, threads: int = 0, fp32: bool = False):
         self.cfg = cfg
+        self._L = lib(fp32)
         self._w = None
         wp = None
         if weights is not None:
             self._w = np.ascontiguousarray(np.asarray(weights, dtype=np.float32))
             wp = self._w.ctypes.data_as(C.POINTER(C.c_float))
-        self._h = lib().ro_create(C.byref(cfg), wp)
+        self._h = self._L.ro_create(C.byref(cfg), wp)
         if threads:
-            lib().ro_set_threads(self._h, threads)
+            self._L.ro_set_threads(self._h, threads)
         self.pwr = [np.zeros(cfg.pwr_len[i]) for i in range(cfg.No)]
         self.ics = [np.zeros(max(cfg.ics_len[i], 1)) for i in range(cfg.No)]
 
     def set_channels(self, chans):
         arr = (C.c_int * len(chans))(*chans)
-        lib().ro_set_channels(self._h, arr, len(chans))
+        self._L.ro_set_channels(self._h, arr, len(chans))
 
     def reset(self):
-        lib().ro_reset(self._h)
+        self._L.ro_reset(self._h)
 
     def process(self, buf: np.ndarray, fft_dir: int = 1):
         """Returns (mask, {i: pwr}, {i: ics}) for products dumped this buffer."""
@@ -203,7 +208,7 @@ class State:
             P[i] = _dptr(self.pwr[i])
             Q[i] = _dptr(self.ics[i])
         buf = np.ascontiguousarray(buf)
-        mask = lib().ro_process(self._h, buf.ctypes.data, fft_dir, P, Q)
+        mask = self._L.ro_process(self._h, buf.ctypes.data, fft_dir, P, Q)
         if mask < 0:
             raise RuntimeError("ro_process failed")
         outs = {i: self.pwr[i].copy() for i in range(self.cfg.No) if mask >> i & 1}
@@ -213,7 +218,7 @@ class State:
 
     def close(self):
         if self._h:
-            lib().ro_destroy(self._h)
+            self._L.ro_destroy(self._h)
             self._h = None
 
     def __del__(self):

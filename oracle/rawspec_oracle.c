@@ -6,6 +6,11 @@
  * Every function cites the reference lines whose behaviour it restates.
  * Arithmetic is IEEE double throughout; twiddles come from libm cos/sin of
  * exactly reduced angles, so the oracle's own error is ~1e-15 relative.
+ *
+ * Built with -DRO_REAL=float (liboracle32.so) the transforms, detection and
+ * sums run in single precision instead: the same algorithm as an FP32 CPU
+ * port, which the CPU baseline times beside the FP64 one.  The oracle the
+ * tests check against is always the double build.
  */
 #include "rawspec_oracle.h"
 
@@ -17,6 +22,11 @@
 #endif
 
 #define RO_ABS(x) ((x) < 0 ? -(x) : (x))
+
+#ifndef RO_REAL
+#define RO_REAL double
+#endif
+typedef RO_REAL ro_real;   /* working precision of transforms and sums */
 
 /* ------------------------------------------------------------------------ */
 /* Validation and derived counts: rawspec_gpu.cu:471-641 (checks, in the    */
@@ -147,19 +157,19 @@ void ro_expand4(uint8_t b, int8_t *re, int8_t *im)
 /* ------------------------------------------------------------------------ */
 typedef struct {
   size_t n;
-  double *tw;   /* forward twiddles exp(-2 pi i m / n), m < n (interleaved) */
+  ro_real *tw;  /* forward twiddles exp(-2 pi i m / n), m < n (interleaved) */
 } ro_fftplan;
 
 static void plan_init(ro_fftplan *p, size_t n)
 {
   size_t m;
   p->n = n;
-  p->tw = (double *)malloc(2 * n * sizeof(double));
+  p->tw = (ro_real *)malloc(2 * n * sizeof(ro_real));
   for (m = 0; m < n; m++) {
     /* angle reduced exactly: 2 pi m / n */
     double a = -2.0 * M_PI * (double)m / (double)n;
-    p->tw[2 * m] = cos(a);
-    p->tw[2 * m + 1] = sin(a);
+    p->tw[2 * m] = (ro_real)cos(a);
+    p->tw[2 * m + 1] = (ro_real)sin(a);
   }
 }
 
@@ -173,14 +183,14 @@ static int is_pow2(size_t n) { return n && !(n & (n - 1)); }
 
 /* twiddle exp(sign * 2 pi i e / n) from the forward table */
 static inline void tw_get(const ro_fftplan *p, size_t e, int sign,
-                          double *wr, double *wi)
+                          ro_real *wr, ro_real *wi)
 {
   e %= p->n;
   *wr = p->tw[2 * e];
   *wi = sign < 0 ? p->tw[2 * e + 1] : -p->tw[2 * e + 1];
 }
 
-static void fft_pow2(const ro_fftplan *p, double *d, int sign)
+static void fft_pow2(const ro_fftplan *p, ro_real *d, int sign)
 {
   size_t n = p->n, i, j, len;
   /* bit reversal */
@@ -189,7 +199,7 @@ static void fft_pow2(const ro_fftplan *p, double *d, int sign)
     for (; j & bit; bit >>= 1) j ^= bit;
     j ^= bit;
     if (i < j) {
-      double tr = d[2 * i], ti = d[2 * i + 1];
+      ro_real tr = d[2 * i], ti = d[2 * i + 1];
       d[2 * i] = d[2 * j]; d[2 * i + 1] = d[2 * j + 1];
       d[2 * j] = tr; d[2 * j + 1] = ti;
     }
@@ -198,11 +208,11 @@ static void fft_pow2(const ro_fftplan *p, double *d, int sign)
     size_t half = len >> 1, step = n / len, k, b;
     for (b = 0; b < n; b += len) {
       for (k = 0; k < half; k++) {
-        double wr, wi;
+        ro_real wr, wi;
         tw_get(p, k * step, sign, &wr, &wi);
-        double *u = d + 2 * (b + k), *v = d + 2 * (b + k + half);
-        double vr = v[0] * wr - v[1] * wi;
-        double vi = v[0] * wi + v[1] * wr;
+        ro_real *u = d + 2 * (b + k), *v = d + 2 * (b + k + half);
+        ro_real vr = v[0] * wr - v[1] * wi;
+        ro_real vi = v[0] * wi + v[1] * wr;
         v[0] = u[0] - vr; v[1] = u[1] - vi;
         u[0] += vr;       u[1] += vi;
       }
@@ -222,8 +232,8 @@ static size_t smallest_factor(size_t n)
 /* Recursive mixed-radix DIT on a strided input: out[k] = sum_m in[m*is] w^(mk)
  * where the base twiddle table belongs to length N_root and this sub-FFT has
  * length n = N_root / tstride. */
-static void fft_rec(const ro_fftplan *p, const double *in, size_t is, double *out,
-                    size_t n, size_t tstride, int sign, double *scratch)
+static void fft_rec(const ro_fftplan *p, const ro_real *in, size_t is, ro_real *out,
+                    size_t n, size_t tstride, int sign, ro_real *scratch)
 {
   size_t r, k, f, m;
   if (n == 1) { out[0] = in[0]; out[1] = in[1]; return; }
@@ -231,11 +241,11 @@ static void fft_rec(const ro_fftplan *p, const double *in, size_t is, double *ou
   if (f == n) {
     /* prime length: direct DFT */
     for (k = 0; k < n; k++) {
-      double sr = 0, si = 0;
+      ro_real sr = 0, si = 0;
       for (r = 0; r < n; r++) {
-        double wr, wi;
+        ro_real wr, wi;
         tw_get(p, ((r * k) % n) * tstride, sign, &wr, &wi);
-        const double *x = in + 2 * r * is;
+        const ro_real *x = in + 2 * r * is;
         sr += x[0] * wr - x[1] * wi;
         si += x[0] * wi + x[1] * wr;
       }
@@ -249,10 +259,10 @@ static void fft_rec(const ro_fftplan *p, const double *in, size_t is, double *ou
     fft_rec(p, in + 2 * r * is, is * f, scratch + 2 * r * m, m, tstride * f,
             sign, scratch + 2 * n);
   for (k = 0; k < n; k++) {
-    double sr = 0, si = 0;
+    ro_real sr = 0, si = 0;
     for (r = 0; r < f; r++) {
-      double wr, wi;
-      const double *y = scratch + 2 * (r * m + (k % m));
+      ro_real wr, wi;
+      const ro_real *y = scratch + 2 * (r * m + (k % m));
       tw_get(p, ((r * k) % n) * tstride, sign, &wr, &wi);
       sr += y[0] * wr - y[1] * wi;
       si += y[0] * wi + y[1] * wr;
@@ -274,15 +284,15 @@ static size_t rec_scratch_len(size_t n)
   return tot + 2;
 }
 
-static void plan_exec(const ro_fftplan *p, double *d, int sign, double *work)
+static void plan_exec(const ro_fftplan *p, ro_real *d, int sign, ro_real *work)
 {
   if (is_pow2(p->n)) {
     fft_pow2(p, d, sign);
   } else {
     size_t n = p->n;
-    double *out = work, *scr = work + 2 * n;
+    ro_real *out = work, *scr = work + 2 * n;
     fft_rec(p, d, 1, out, n, 1, sign, scr);
-    memcpy(d, out, 2 * n * sizeof(double));
+    memcpy(d, out, 2 * n * sizeof(ro_real));
   }
 }
 
@@ -294,12 +304,17 @@ static size_t plan_work_len(size_t n)
 void ro_fft(double *data, size_t n, int sign)
 {
   ro_fftplan p;
-  double *work;
+  ro_real *x, *work;
+  size_t m;
   if (n <= 1) return;
   plan_init(&p, n);
-  work = (double *)malloc((plan_work_len(n) + 2) * sizeof(double));
-  plan_exec(&p, data, sign, work);
+  x = (ro_real *)malloc(2 * n * sizeof(ro_real));
+  work = (ro_real *)malloc((plan_work_len(n) + 2) * sizeof(ro_real));
+  for (m = 0; m < 2 * n; m++) x[m] = (ro_real)data[m];
+  plan_exec(&p, x, sign, work);
+  for (m = 0; m < 2 * n; m++) data[m] = (double)x[m];
   free(work);
+  free(x);
   plan_free(&p);
 }
 
@@ -328,7 +343,7 @@ void ro_dft_naive(const double *in, double *out, size_t n, int sign)
 struct ro_state {
   ro_cfg c;
   ro_fftplan plan[RO_MAX_OUTPUTS];
-  double *acc[RO_MAX_OUTPUTS];   /* integration in progress, host layout */
+  ro_real *acc[RO_MAX_OUTPUTS];  /* integration in progress, host layout */
   double *w;                     /* Nant antenna weights */
   unsigned long inbuf_count;
   int *chans;
@@ -343,7 +358,7 @@ ro_state *ro_create(const ro_cfg *c, const float *weights)
   s->c = *c;
   for (i = 0; i < c->No; i++) {
     plan_init(&s->plan[i], c->Nts[i]);
-    s->acc[i] = (double *)calloc(c->pwr_len[i], sizeof(double));
+    s->acc[i] = (ro_real *)calloc(c->pwr_len[i], sizeof(ro_real));
   }
   s->w = (double *)calloc(c->Nant, sizeof(double));
   for (i = 0; i < c->Nant; i++) {
@@ -375,7 +390,7 @@ void ro_reset(ro_state *s)
   unsigned i;
   /* rawspec_gpu.cu:1762-1792: partial integrations dropped, count to 0 */
   for (i = 0; i < s->c.No; i++)
-    memset(s->acc[i], 0, s->c.pwr_len[i] * sizeof(double));
+    memset(s->acc[i], 0, s->c.pwr_len[i] * sizeof(ro_real));
   s->inbuf_count = 0;
 }
 
@@ -397,7 +412,7 @@ void ro_set_threads(ro_state *s, int nthreads) { s->nthreads = nthreads; }
  * device buffer [Nc][Nb*Ntpb][Np][re,im] (rawspec_gpu.cu:133-148 addressing,
  * :1454-1460 layout). */
 static void unpack(const ro_cfg *c, const void *buf, unsigned ch, size_t t0,
-                   unsigned p, size_t nt, double *x)
+                   unsigned p, size_t nt, ro_real *x)
 {
   size_t m, chan_samples = (size_t)c->Nb * c->Ntpb;
   size_t base = ((size_t)ch * chan_samples + t0) * c->Np + p; /* complex idx */
@@ -405,15 +420,15 @@ static void unpack(const ro_cfg *c, const void *buf, unsigned ch, size_t t0,
     const int16_t *b = (const int16_t *)buf;
     for (m = 0; m < nt; m++) {
       size_t e = 2 * (base + m * c->Np);
-      x[2 * m] = ro_snorm16(b[e]);
-      x[2 * m + 1] = ro_snorm16(b[e + 1]);
+      x[2 * m] = (ro_real)ro_snorm16(b[e]);
+      x[2 * m + 1] = (ro_real)ro_snorm16(b[e + 1]);
     }
   } else {
     const int8_t *b = (const int8_t *)buf;
     for (m = 0; m < nt; m++) {
       size_t e = 2 * (base + m * c->Np);
-      x[2 * m] = ro_snorm8(b[e]);
-      x[2 * m + 1] = ro_snorm8(b[e + 1]);
+      x[2 * m] = (ro_real)ro_snorm8(b[e]);
+      x[2 * m + 1] = (ro_real)ro_snorm8(b[e + 1]);
     }
   }
 }
@@ -423,26 +438,26 @@ static void unpack(const ro_cfg *c, const void *buf, unsigned ch, size_t t0,
  * :1659-1691 (the two-copy FFT shift: D[k] lands at (k + Nt/2) mod Nt).
  * Pol-mode mapping follows the code: +4 -> XX,YY,Re XY*,Im XY*
  * (store_callback_pol0/pol1[_conj]); -4 -> I,Q,U,V (…_iquv[_conj]). */
-static void detect_acc(const ro_cfg *c, int i, const double *X0,
-                       const double *X1, double *acc_cd, size_t qstride)
+static void detect_acc(const ro_cfg *c, int i, const ro_real *X0,
+                       const ro_real *X1, ro_real *acc_cd, size_t qstride)
 {
   size_t nt = c->Nts[i], k;
   size_t half = nt / 2;
   int mode = c->Npolout[i];
-  double g = c->input_conjugated ? -1.0 : 1.0;
+  ro_real g = c->input_conjugated ? -1 : 1;
   for (k = 0; k < nt; k++) {
     size_t j = (k + half) % nt;
-    double a = X0[2 * k], b = X0[2 * k + 1];
-    double p0 = a * a + b * b;
+    ro_real a = X0[2 * k], b = X0[2 * k + 1];
+    ro_real p0 = a * a + b * b;
     if (mode == 1) {
-      double p = p0;
+      ro_real p = p0;
       if (X1) p += X1[2 * k] * X1[2 * k] + X1[2 * k + 1] * X1[2 * k + 1];
       acc_cd[j] += p;
     } else {
-      double cr = X1[2 * k], ci = X1[2 * k + 1];
-      double p1 = cr * cr + ci * ci;
-      double xre = a * cr + b * ci;          /* Re(X0 conj X1) */
-      double xim = g * (b * cr - a * ci);    /* Im(X0 conj X1), conj flips */
+      ro_real cr = X1[2 * k], ci = X1[2 * k + 1];
+      ro_real p1 = cr * cr + ci * ci;
+      ro_real xre = a * cr + b * ci;         /* Re(X0 conj X1) */
+      ro_real xim = g * (b * cr - a * ci);   /* Im(X0 conj X1), conj flips */
       if (mode == 4) {
         acc_cd[j] += p0;
         acc_cd[qstride + j] += p1;
@@ -480,8 +495,8 @@ int ro_process(ro_state *s, const void *buf, int fft_dir,
 #pragma omp parallel num_threads(nth)
 #endif
     {
-      double *x0 = (double *)malloc((2 * nt + wl + 2) * sizeof(double));
-      double *x1 = (double *)malloc((2 * nt + wl + 2) * sizeof(double));
+      ro_real *x0 = (ro_real *)malloc((2 * nt + wl + 2) * sizeof(ro_real));
+      ro_real *x1 = (ro_real *)malloc((2 * nt + wl + 2) * sizeof(ro_real));
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
@@ -490,7 +505,7 @@ int ro_process(ro_state *s, const void *buf, int fft_dir,
         unsigned g = (unsigned)(t % ngroups);
         unsigned d = (nis == 1) ? g : 0;
         size_t qstride = (size_t)c->Nc * nt;
-        double *acc_cd = s->acc[i] + ((size_t)d * q * c->Nc + ch) * nt;
+        ro_real *acc_cd = s->acc[i] + ((size_t)d * q * c->Nc + ch) * nt;
         unsigned sp;
         for (sp = g * na_buf; sp < (g + 1) * na_buf; sp++) {
           unpack(c, buf, ch, (size_t)sp * nt, 0, nt, x0);
@@ -507,7 +522,10 @@ int ro_process(ro_state *s, const void *buf, int fft_dir,
     }
     (void)nds;
     if (s->inbuf_count % nis == 0) {    /* :1572 dump */
-      if (pwr && pwr[i]) memcpy(pwr[i], s->acc[i], c->pwr_len[i] * sizeof(double));
+      if (pwr && pwr[i]) {
+        size_t e;
+        for (e = 0; e < c->pwr_len[i]; e++) pwr[i][e] = (double)s->acc[i][e];
+      }
       if (c->incoherently_sum && ics && ics[i]) {
         /* ICS (:321-340 at Nbc = Nc/Nant): weighted sum over antennas of the
          * integrated, shifted spectra; layout [Nds][|P|][Nc/Nant][Nt]. */
@@ -517,14 +535,14 @@ int ro_process(ro_state *s, const void *buf, int fft_dir,
           for (qq = 0; qq < q; qq++)
             for (cc = 0; cc < ncpa; cc++)
               for (j = 0; j < nt; j++) {
-                double sum = 0;
+                ro_real sum = 0;
                 for (a = 0; a < c->Nant; a++)
-                  sum += s->w[a] *
+                  sum += (ro_real)s->w[a] *
                          s->acc[i][((dd * q + qq) * c->Nc + a * ncpa + cc) * nt + j];
                 ics[i][((dd * q + qq) * ncpa + cc) * nt + j] = sum;
               }
       }
-      memset(s->acc[i], 0, c->pwr_len[i] * sizeof(double));   /* :1711-1719 */
+      memset(s->acc[i], 0, c->pwr_len[i] * sizeof(ro_real));  /* :1711-1719 */
       mask |= 1 << i;
     }
   }

@@ -13,6 +13,10 @@
  *                    schedule, FFT shift, ICS       1517-1747
  *   ro_gen_*         seeded synthetic GUPPI blocks  (no reference counterpart: the
  *                                                   generator behind the test inputs)
+ *
+ * The Makefile also builds the same source with -DRO_REAL=float
+ * (liboracle32.so): the algorithm in single precision, an FP32 CPU port that
+ * the CPU baseline times.  It is never the checker; the API stays double.
  */
 #ifndef RAWSPEC_ORACLE_H_
 #define RAWSPEC_ORACLE_H_

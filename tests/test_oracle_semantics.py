@@ -227,3 +227,26 @@ def test_validation_derivations():
     assert c.Nb == 4 and c.Nb_host == 4 and c.Ntmax == 4096
     assert c.Nbc == 1                                # largest factor <= 10 of 10 -> 10/10
     assert c.Nis[0] == 2 and c.Nds[1] == 1 and c.Nis[1] == 4
+
+
+@pytest.mark.parametrize("nts,pols", [([1024, 8], [-4, 1]), ([1000, 100], [4, 1])])
+def test_single_precision_port_tracks_the_oracle(nts, pols):
+    """liboracle32.so is the same code in single precision (the FP32 CPU
+    port the CPU baseline times): its dumps agree with the FP64 oracle's to
+    FP32 rounding, and differ from them (it really computes in FP32)."""
+    Nc, Ntpb = 2, 4000 if nts[0] == 1000 else 4096
+    cfg = O.make_cfg(No=2, Np=2, Nc=Nc, Ntpb=Ntpb, Nbps=8, Npolout=pols, Nts=nts, Nas=[2, 4])
+    assert O.derive(cfg) == 0
+    blk = O.gen_block(Nc, Ntpb, 2, 8, 5, 8.0, 127, 0, 1, 20.0, 100.0, float(nts[0]))
+    buf = O.device_layout([blk] * cfg.Nb, Nc, Ntpb, 2)
+    s64, s32 = O.State(cfg), O.State(cfg, fp32=True)
+    _, r64, _ = s64.process(buf, 1)
+    _, r32, _ = s32.process(buf, 1)
+    s64.close()
+    s32.close()
+    assert sorted(r64) == sorted(r32) == [0, 1]
+    for i in r64:
+        a, b = r64[i], r32[i]
+        tot = np.abs(a).max()
+        assert np.abs(a - b).max() <= 1e-5 * tot, i
+        assert not np.array_equal(a, b), i
