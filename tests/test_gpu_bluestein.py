@@ -83,3 +83,16 @@ def test_longer_lengths_refused():
     import rawspec_amd as R
     with pytest.raises(RuntimeError):
         R.Spectrometer(Nc=1, Ntpb=3000 * 4, Nts=[3000], Nas=[1], Np=2, Nbps=8, Npolout=[1])
+
+
+def test_incoherent_sum_and_complex4():
+    # 4 antennas x 2 channels, weighted incoherent sum of 768- and 24-point
+    # products; then 4-bit complex input expanded on the device
+    w = [0.5 + 0.25 * a for a in range(4)]
+    res, cfg, _ = run_case(Nc=8, Nant=4, Ntpb=768 * 16, Nts=[768, 24], Nas=[8, 64],
+                           Npolout=[1, -4], nbuf=2, ics=True, Aws=w,
+                           tone=(2, 20.0, 50.0, 768.0))
+    check_results(res, cfg, tol=PLAIN_TOL, ics=True)
+    res, cfg, _ = run_case(Nc=4, Ntpb=1000 * 16, Nts=[1000, 40], Nas=[8, 100],
+                           Npolout=[-4, 1], nbuf=2, expand4=True)
+    check_results(res, cfg, tol=PLAIN_TOL)
