@@ -45,7 +45,7 @@ constexpr int small_lds_f2() {
 // conjugated (an inverse FFT), and conjugated times c; bins k >= NT are
 // zero and dropped.
 template <int N, int NBPS, int NP, int PMODE, bool BS = false>
-__global__ __launch_bounds__(wg_for(N)) __attribute__((amdgpu_waves_per_eu(2)))
+__global__ __launch_bounds__(wg_for(N)) __attribute__((amdgpu_waves_per_eu(BS ? 1 : 2)))
 void k_small(SmallArgs a) {
   constexpr int E = elems_for(N);
   constexpr int T = N / E;
@@ -122,11 +122,16 @@ void k_small(SmallArgs a) {
 #pragma unroll
       for (int p = 0; p < NP; ++p) v[p][m] = o[p];
     }
+    // Bluestein tables are read per spectrum (L1 and L2 hold them): an
+    // opaque zero in the index keeps the compiler from holding all 3 E of
+    // their values in registers across the spectrum loop, which spills
+    unsigned zt = 0;
+    if constexpr (BS) asm volatile("v_mov_b32 %0, 0" : "=v"(zt));
     if constexpr (BS) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const unsigned n = (unsigned)(j + m * T);
-        const float2 cn = n < NT ? a.chirp[n] : make_float2(0.0f, 0.0f);
+        const float2 cn = n < NT ? a.chirp[n + zt] : make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int p = 0; p < NP; ++p) v[p][m] = cmul(v[p][m], cn);
       }
@@ -136,18 +141,23 @@ void k_small(SmallArgs a) {
     if constexpr (BS) {
 #pragma unroll
       for (int m = 0; m < E; ++m) {
-        const float2 bk = a.bsk[j + m * T];
+        const float2 bk = a.bsk[j + m * T + zt];
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
           const float2 z = cmul(v[p][m], bk);
           v[p][m] = make_float2(z.x, -z.y);
         }
       }
-      team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp);
+      // the second transform reads the twiddle table through an opaque
+      // offset, so its loads are not merged with the first's (which would
+      // keep those values live in between)
+      unsigned zw;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(zw));
+      team_fft<N, E, NP, TwTab<N>::TWN, (T <= 64)>(v, lds + slot * team_stride<N, NP>(), j, twp + zw);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const unsigned k = (unsigned)(j + m * T);
-        const float2 ck = k < NT ? a.chirp[k] : make_float2(0.0f, 0.0f);
+        const float2 ck = k < NT ? a.chirp[k + zt] : make_float2(0.0f, 0.0f);
 #pragma unroll
         for (int p = 0; p < NP; ++p) v[p][m] = cmul(make_float2(v[p][m].x, -v[p][m].y), ck);
       }
