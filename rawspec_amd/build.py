@@ -74,16 +74,40 @@ def san_libs(sanitize):
     return libs
 
 
+RESOURCE_FIELDS = ("VGPRs", "AGPRs", "ScratchSize [bytes/lane]", "Occupancy [waves/SIMD]")
+
+
+def kernel_resources(remarks: str):
+    """(kernel, {field: int}) per kernel from the compiler's
+    kernel-resource-usage remarks."""
+    out, cur = [], None
+    for line in remarks.splitlines():
+        if "remark:" not in line:
+            continue
+        body = line.split("remark:", 1)[1].split("[-Rpass")[0].strip()
+        if body.startswith("Function Name:"):
+            cur = (body.split(":", 1)[1].strip(), {})
+            out.append(cur)
+        elif cur is not None and ":" in body:
+            key, val = body.rsplit(":", 1)
+            if key.strip() in RESOURCE_FIELDS:
+                cur[1][key.strip()] = int(val.strip())
+    return out
+
+
 def compile_one(src, build_dir=BUILD, defines=(), sanitize=None):
+    """Compiles one source into build_dir; returns (object, per-kernel
+    resources of a HIP source)."""
     path = os.path.join(CSRC, src)
     out = os.path.join(build_dir, os.path.basename(src) + ".o")
     dflags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-c", *HIP_FLAGS, *dflags, "-x", "hip", path, "-o", out]
+        cmd = [HIPCC, "-c", *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", *dflags, "-x",
+               "hip", path, "-o", out]
     else:
         cmd = [CC, "-c", *C_FLAGS, *san_flags(sanitize), *dflags, path, "-o", out]
-    _run(cmd)
-    return out
+    r = _run(cmd)
+    return out, (kernel_resources(r.stderr) if src.endswith(".hip") else [])
 
 
 def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
@@ -100,7 +124,17 @@ def build(jobs: int | None = None, verbose: bool = False, out: str = LIB,
     srcs = HIP_SOURCES + C_SOURCES
     jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, build_dir, defines, sanitize), srcs))
+        done = list(ex.map(lambda s: compile_one(s, build_dir, defines, sanitize), srcs))
+    objs = [o for o, _ in done]
+    # registers, scratch and occupancy of every kernel, as the compiler
+    # reports them (the tests check that no kernel needs scratch memory,
+    # which the runtime would allocate outside rawspec_amd_plan's figure)
+    with open(os.path.join(build_dir, "kernel_resources.tsv"), "w") as f:
+        f.write("kernel\t" + "\t".join(RESOURCE_FIELDS) + "\n")
+        for _, res in done:
+            for name, fields in res:
+                f.write(name + "\t" + "\t".join(str(fields.get(k, -1)) for k in RESOURCE_FIELDS)
+                        + "\n")
     tmp = out + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", "-o", tmp, *objs,
           *san_libs(sanitize), "-Wl,--no-undefined", "-Wl,-soname,librawspec.so"])

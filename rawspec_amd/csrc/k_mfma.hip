@@ -58,9 +58,13 @@ __device__ __forceinline__ void fs_b_column(const float4 *y, float s1,
 }
 
 // Wavefronts per SIMD the loop variant of k_fs_b_mfma is built for (the
-// running sum is live across its transforms).
+// running sum is live across its transforms).  At two (256 VGPRs) the
+// four-component modes spill 76 bytes per lane, for which the runtime keeps
+// scratch memory for every wave the device can hold; at one the spare
+// registers hold them, at the same speed (0.348 ms per 256 MiB buffer of
+// the 64-channel 2^20 product either way).
 #ifndef RSK_FSB_LOOP_WAVES
-#define RSK_FSB_LOOP_WAVES 2
+#define RSK_FSB_LOOP_WAVES 1
 #endif
 
 // ONE: the dump group is a single spectrum (na_buf == 1, e.g. one 2^20

@@ -190,3 +190,21 @@ def test_plan_bytes_for_the_64_channel_three_product_workload(rlib):
     assert fixed + tw <= dev.value <= fixed + tw + 64 * MiB, dev.value
     # pinned host: three dump buffers (blocks are allocated too: h_blkbufs NULL)
     assert host.value == 1024 * MiB + 8 * MiB + 1 * MiB + 2 * 256 * MiB // 2
+
+
+def test_no_kernel_needs_scratch_memory():
+    """Every kernel of the built library keeps its values in registers: the
+    HIP runtime would otherwise allocate scratch memory for every wave the
+    device can hold (hundreds of MiB) outside what rawspec_amd_plan()
+    reports and admission control counts.  Read from the compiler's
+    resource report that the build writes."""
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "rawspec_amd", "_build", "kernel_resources.tsv")
+    if not os.path.exists(path):
+        pytest.skip("the library was not built here")
+    with open(path) as f:
+        head = f.readline().rstrip("\n").split("\t")
+        rows = [dict(zip(head, line.rstrip("\n").split("\t"))) for line in f]
+    assert len(rows) > 100
+    spill = [r["kernel"] for r in rows if int(r["ScratchSize [bytes/lane]"]) != 0]
+    assert not spill, spill[:5]
