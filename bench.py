@@ -448,13 +448,17 @@ def main(argv=None):
                 for k, (n, ms) in prof.items()}
         if mfma:
             # the matrix-core engine at its 1e-3 bar runs the 1024-point
-            # product (3072 spectra per dump) on unsplit FP16 operands and
-            # keeps the 2^20 product's intermediate in 24-bit floats; both
-            # four-step passes and the 8-point product stay on the vector
-            # units, where they are faster
+            # product (3072 spectra per dump) on unsplit FP16 operands, with
+            # the 8-point product (128 spectra per dump) computed in FP32 by
+            # the same kernel from the same read of the samples, and keeps
+            # the 2^20 product's intermediate in 24-bit floats; both
+            # four-step passes stay on the vector units, where they are
+            # faster
             names = {k.split(".")[1] for k in kern if k.endswith(".N1024")}
-            assert names == {"k_small_mfma_u", "k_reduce"} or names == {"k_small_mfma_u"}, \
+            assert names == {"k_small_mfma_u8", "k_reduce"} or names == {"k_small_mfma_u8"}, \
                 f"matrix-core leg ran {sorted(names)} for the 1024-point product"
+            assert not any(k.endswith(".N8") for k in kern), \
+                f"matrix-core leg ran {sorted(kern)}: the 8-point product has a kernel of its own"
             assert "p0.k_fs_a.N1048576" in kern and "p0.k_fs_b.N1048576" in kern, \
                 f"matrix-core leg ran {sorted(kern)}"
         gbs = a.steps * buf_bytes / dt / 1e9

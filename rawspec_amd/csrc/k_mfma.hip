@@ -266,6 +266,112 @@ __global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2
   }
 }
 
+// ---- an 8-point product fused into the unsplit 1024-point kernel ----------
+// Forward 8-point DFT in place, natural order: radix 2, decimation in
+// frequency, W8 = exp(-2 pi i / 8).
+__device__ __forceinline__ void dft8(float2 (&x)[8]) {
+  constexpr float r = 0.70710678118654752f;
+  const float2 a0 = cadd(x[0], x[4]), a1 = cadd(x[1], x[5]);
+  const float2 a2 = cadd(x[2], x[6]), a3 = cadd(x[3], x[7]);
+  const float2 b0 = csub(x[0], x[4]), d1 = csub(x[1], x[5]);
+  const float2 d2 = csub(x[2], x[6]), d3 = csub(x[3], x[7]);
+  // times W8^1 = (1 - i) r, W8^2 = -i, W8^3 = (-1 - i) r
+  const float2 b1 = make_float2((d1.x + d1.y) * r, (d1.y - d1.x) * r);
+  const float2 b2 = make_float2(d2.y, -d2.x);
+  const float2 b3 = make_float2((d3.y - d3.x) * r, -(d3.x + d3.y) * r);
+  // the two 4-point halves: even outputs from a, odd outputs from b
+  const float2 c0 = cadd(a0, a2), c1 = cadd(a1, a3), c2 = csub(a0, a2), e3 = csub(a1, a3);
+  const float2 c3 = make_float2(e3.y, -e3.x);
+  const float2 g0 = cadd(b0, b2), g1 = cadd(b1, b3), g2 = csub(b0, b2), f3 = csub(b1, b3);
+  const float2 g3 = make_float2(f3.y, -f3.x);
+  x[0] = cadd(c0, c1);
+  x[4] = csub(c0, c1);
+  x[2] = cadd(c2, c3);
+  x[6] = csub(c2, c3);
+  x[1] = cadd(g0, g1);
+  x[5] = csub(g0, g1);
+  x[3] = cadd(g2, g3);
+  x[7] = csub(g2, g3);
+}
+
+template <int CTRL, int BANKS>
+__device__ __forceinline__ float dpp_f(float old, float src) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                               __builtin_bit_cast(int, src),
+                                                               CTRL, 0xf, BANKS, false));
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_q(float src) {   // quad_perm: every lane written
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, src),
+                                                               CTRL, 0xf, 0xf, false));
+}
+
+// One step of a cross-lane reduce-scatter between lanes l and l ^ D: x and
+// y are values k and k + n/2 of a lane's n partial sums; afterwards x holds,
+// summed over both lanes, value k where lane bit D is clear and value
+// k + n/2 where it is set.
+template <int D>
+__device__ __forceinline__ void halve(float &x, float y, int lane) {
+  if constexpr (D == 32 || D == 16) {
+    // gfx950 lane swaps: the upper 32 lanes (D = 32) or the odd 16-lane rows
+    // (D = 16) of the first operand trade places with the lower lanes / even
+    // rows of the second
+    const auto r = D == 32 ? __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(unsigned, x),
+                                                              __builtin_bit_cast(unsigned, y), false, false)
+                           : __builtin_amdgcn_permlane16_swap(__builtin_bit_cast(unsigned, x),
+                                                              __builtin_bit_cast(unsigned, y), false, false);
+    // the second result passes through an empty asm: with both results
+    // bit-cast to float, this compiler's optimiser replaces the second by
+    // the first (fadd of one value with itself)
+    unsigned int r1 = r[1];
+    asm volatile("" : "+v"(r1));
+    x = __builtin_bit_cast(float, r[0]) + __builtin_bit_cast(float, r1);
+  } else if constexpr (D == 8 || D == 4) {
+    // row shifts; each writes only the lanes (4-lane banks) it serves
+    constexpr int lo = D == 8 ? 0x3 : 0x5;
+    const float m1 = dpp_f<0x100 + D, lo>(y, x);          // row_shl:D
+    const float m2 = dpp_f<0x110 + D, 0xf & ~lo>(x, y);   // row_shr:D
+    x = m1 + m2;
+  } else {
+    static_assert(D == 2 || D == 1, "lane distance");
+    const bool up = (lane & D) != 0;
+    const float keep = up ? y : x, send = up ? x : y;
+    x = keep + dpp_q<D == 2 ? 0x4e : 0xb1>(send);        // quad_perm lane ^ D
+  }
+}
+
+// Sum of V partial sums v[0..V) over the 64 lanes of the wavefront: value o
+// ends in lanes (o << (6 - log2 V)) .. (o + 1 << (6 - log2 V)) - 1.
+template <int V>
+__device__ __forceinline__ float wave_reduce(float (&v)[V], int lane) {
+  static_assert(V == 32 || V == 8, "32 or 8 partial sums");
+#pragma unroll
+  for (int k = 0; k < V / 2; ++k) halve<32>(v[k], v[k + V / 2], lane);
+#pragma unroll
+  for (int k = 0; k < V / 4; ++k) halve<16>(v[k], v[k + V / 4], lane);
+#pragma unroll
+  for (int k = 0; k < V / 8; ++k) halve<8>(v[k], v[k + V / 8], lane);
+  float x = v[0];
+  if constexpr (V == 32) {
+    halve<4>(x, v[2], lane);
+    float y = v[1];
+    halve<4>(y, v[3], lane);
+    halve<2>(x, y, lane);
+    return x + dpp_q<0xb1>(x);                            // lanes 2m, 2m + 1
+  } else {
+    x += dpp_f<0x141, 0xf>(0.0f, x);                      // row_half_mirror: l, 7 - l
+    x += dpp_q<0xb1>(x);
+    return x + dpp_q<0x4e>(x);                            // 8-lane groups
+  }
+}
+
+// Staging of one spectrum's samples for the fused 8-point product: sample n
+// (its two pols' FP16 pairs) at uint2 n + 2 (n >> 4), so the 16-byte reads
+// of eight consecutive lanes, each lane one spectrum of 8 samples, fall in
+// different banks.
+constexpr int kF8Tile = 1024 + 2 * 64;                 // uint2 per wavefront
+__device__ __forceinline__ int f8_slot(int n) { return n + 2 * (n >> 4); }
+
 // k_small_mfma_u: k_small_mfma's contract for int8 samples with unsplit
 // FP16 operands (mfma_dft.h, dft1024u): the samples become FP16 pairs by a
 // byte permute (no FP32 conversion), the inverse transform uses conjugated
@@ -274,16 +380,28 @@ __global__ __launch_bounds__(64 * kMfmaWaves, (NBPS == 16 && PMODE != 1) ? 1 : 2
 // matrix (32 registers) stays in registers for the whole chunk, the
 // twiddles are read from LDS per transform; the samples are loaded two
 // spectra ahead of the transforms.
-template <int PMODE>
+//
+// F8: the same wavefronts also compute an 8-point product whose dumps
+// integrate 128 spectra, i.e. one dump per 1024-sample spectrum of this
+// product, from the samples already loaded: each spectrum's FP16 pairs go
+// through a per-wavefront LDS tile, every lane reads two 8-sample spectra
+// back, transforms both pols in FP32 (exact inputs, the FP32 vector
+// engine's precision) and detects; the wavefront's 32 (or 8) partial sums
+// are reduced across lanes and written, FFT-shifted, to a.f8_out.  The
+// transform is always forward: for the inverse, bin k is forward bin -k and
+// the cross term's sign is folded into gam as for the 1024-point product.
+template <int PMODE, bool F8>
 __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a, unsigned items) {
   constexpr int N = 1024;
   constexpr int Q = nq<PMODE>();
   constexpr int kStBytes = kMfmaWaves * Q * kSmRow * 4;
-  constexpr int kLds = kStBytes > (int)sizeof(Dft1024U) ? kStBytes : (int)sizeof(Dft1024U);
+  constexpr int kLoopBytes = (int)sizeof(Dft1024U) + (F8 ? kMfmaWaves * kF8Tile * 8 : 0);
+  constexpr int kLds = kStBytes > kLoopBytes ? kStBytes : kLoopBytes;
   __shared__ __attribute__((aligned(16))) unsigned char lds_raw[kLds];
   Dft1024U &K = *reinterpret_cast<Dft1024U *>(lds_raw);
   float *st = reinterpret_cast<float *>(lds_raw);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint2 *t8 = reinterpret_cast<uint2 *>(lds_raw + sizeof(Dft1024U)) + wv * kF8Tile;
   const int col = lane & 31, h = lane >> 5;
   const unsigned item = blockIdx.x * kMfmaWaves + wv;
   const bool valid = item < items;
@@ -297,11 +415,15 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
   // cross term's sign is the input-conjugation sign alone
   dft1024u_consts(K, a.tw, 64 * kMfmaWaves, a.isign);
   __syncthreads();
+  // the DFT matrix: held for the whole chunk, or (F8, whose 8-point work
+  // needs the registers) read from LDS per spectrum
   h8v bre[4], bim[4];
+  if constexpr (!F8) {
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    bre[kb] = K.b[0][kb][lane];
-    bim[kb] = K.b[1][kb][lane];
+    for (int kb = 0; kb < 4; ++kb) {
+      bre[kb] = K.b[0][kb][lane];
+      bim[kb] = K.b[1][kb][lane];
+    }
   }
   const float gam = a.gam * a.isign;
 
@@ -315,13 +437,15 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
                             col + 32 * 4 * h;
   const unsigned nspec = valid ? a.chunk : 0u;
   // samples of spectrum s in raw, of s + 1 in nxt (loads two spectra ahead
-  // of the transforms: one spectrum's compute does not cover HBM latency)
+  // of the transforms: one spectrum's compute does not cover HBM latency;
+  // F8, with the 8-point work besides, loads one spectrum ahead into raw)
+  constexpr unsigned AHEAD = F8 ? 1 : 2;
   unsigned int raw[16], nxt[16];
   if (nspec) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) raw[e] = src[32 * (8 * (e >> 2) + (e & 3))];
   }
-  if (nspec > 1) {
+  if (!F8 && nspec > 1) {
 #pragma unroll
     for (int e = 0; e < 16; ++e) nxt[e] = src[N + 32 * (8 * (e >> 2) + (e & 3))];
   }
@@ -332,24 +456,35 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
       const unsigned int t = raw[e] ^ 0x80808080u;
-      const h2v p0 = __builtin_bit_cast(h2v, i8_pair_f16<0>(t));
-      const h2v p1 = __builtin_bit_cast(h2v, i8_pair_f16<1>(t));
+      const unsigned int u0 = i8_pair_f16<0>(t), u1 = i8_pair_f16<1>(t);
+      const h2v p0 = __builtin_bit_cast(h2v, u0);
+      const h2v p1 = __builtin_bit_cast(h2v, u1);
       ah[0][e >> 2][2 * (e & 3)] = p0[0];
       ah[0][e >> 2][2 * (e & 3) + 1] = p0[1];
       ah[1][e >> 2][2 * (e & 3)] = p1[0];
       ah[1][e >> 2][2 * (e & 3) + 1] = p1[1];
+      if constexpr (F8) t8[f8_slot(col + 32 * (8 * (e >> 2) + 4 * h + (e & 3)))] = make_uint2(u0, u1);
     }
+    if constexpr (!F8) {
 #pragma unroll
-    for (int e = 0; e < 16; ++e) raw[e] = nxt[e];
-    if (s + 2 < nspec) {
-      const unsigned int *nx = src + (unsigned long long)(s + 2) * N;
+      for (int e = 0; e < 16; ++e) raw[e] = nxt[e];
+    }
+    if (s + AHEAD < nspec) {
+      const unsigned int *nx = src + (unsigned long long)(s + AHEAD) * N;
 #pragma unroll
-      for (int e = 0; e < 16; ++e) nxt[e] = nx[32 * (8 * (e >> 2) + (e & 3))];
+      for (int e = 0; e < 16; ++e) (F8 ? raw : nxt)[e] = nx[32 * (8 * (e >> 2) + (e & 3))];
     }
     // the twiddles are read from LDS per transform (an opaque zero in the
     // index keeps the compiler from holding all 32 registers of them)
     int z;
     asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+    if constexpr (F8) {
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) {
+        bre[kb] = K.b[0][kb][lane + z];
+        bim[kb] = K.b[1][kb][lane + z];
+      }
+    }
     // one polarisation after the other (scheduling barriers): interleaving
     // the two transforms would double their live registers
     f16v xr0, xi0, xr1, xi1;
@@ -360,6 +495,61 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int r = 0; r < 16; ++r) detect_raw<PMODE>(acc[r], xr0[r], xi0[r], xr1[r], xi1[r]);
+    if constexpr (F8) {
+      // the tile is this wavefront's own: its LDS operations complete in
+      // order, so a fence for the compiler is all the exchange needs
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      float a8[8][Q];
+#pragma unroll
+      for (int b = 0; b < 8; ++b)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) a8[b][q] = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int m = lane + 64 * t;                      // 8-point spectrum of this one
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(t8 + f8_slot(8 * m));
+        uint4 w[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = p4[k];
+        float2 x0[8], x1[8];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const h2v s00 = __builtin_bit_cast(h2v, w[k].x), s01 = __builtin_bit_cast(h2v, w[k].y);
+          const h2v s10 = __builtin_bit_cast(h2v, w[k].z), s11 = __builtin_bit_cast(h2v, w[k].w);
+          x0[2 * k] = make_float2((float)s00[0], (float)s00[1]);
+          x1[2 * k] = make_float2((float)s01[0], (float)s01[1]);
+          x0[2 * k + 1] = make_float2((float)s10[0], (float)s10[1]);
+          x1[2 * k + 1] = make_float2((float)s11[0], (float)s11[1]);
+        }
+        dft8(x0);
+        dft8(x1);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) detect_raw<PMODE>(a8[b], x0[b].x, x0[b].y, x1[b].x, x1[b].y);
+        __builtin_amdgcn_sched_barrier(0);             // one spectrum's samples live at a time
+      }
+      constexpr int V = 8 * Q;
+      float v[V];
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        detect_finish<PMODE>(a8[b], gam);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) v[q * 8 + b] = a8[b][q];
+      }
+      const float sum = wave_reduce<V>(v, lane);
+      constexpr int SH = V == 32 ? 1 : 3;
+      if ((lane & ((1 << SH) - 1)) == 0) {
+        const int o = lane >> SH, q = o >> 3, b = o & 7;
+        const int k = a.isign > 0.0f ? b : (8 - b) & 7;
+        const unsigned long long d = (unsigned long long)g * a.na_buf + (unsigned long long)chunk * a.chunk + s;
+        a.f8_out[((d * Q + q) * a.Nc + c) * 8 + ((k + 4) & 7)] = sum * a.scale;
+      }
+      // the next spectrum's stores to the tile follow these loads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 
   __syncthreads();   // every wavefront is done with the constants
@@ -390,20 +580,28 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
   }
 }
 
-template <int PMODE>
+template <int PMODE, bool F8>
 static int go_small_mfma_u(const SmallArgs &a, unsigned items, hipStream_t s) {
   const unsigned nb = (items + kMfmaWaves - 1) / kMfmaWaves;
-  hipLaunchKernelGGL((k_small_mfma_u<PMODE>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a, items);
+  hipLaunchKernelGGL((k_small_mfma_u<PMODE, F8>), dim3(nb), dim3(64 * kMfmaWaves), 0, s, a, items);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_small_mfma_u(int n, int nbps, int np, int pmode, const SmallArgs &a, unsigned items,
                         hipStream_t s) {
   if (!small_mfma_ok(n, np) || nbps != 8 || items == 0 || a.gpw != 1) return 2;
+  if (a.f8_out) {
+    switch (pmode) {
+      case 1: return go_small_mfma_u<1, true>(a, items, s);
+      case 4: return go_small_mfma_u<4, true>(a, items, s);
+      case -4: return go_small_mfma_u<-4, true>(a, items, s);
+    }
+    return 2;
+  }
   switch (pmode) {
-    case 1: return go_small_mfma_u<1>(a, items, s);
-    case 4: return go_small_mfma_u<4>(a, items, s);
-    case -4: return go_small_mfma_u<-4>(a, items, s);
+    case 1: return go_small_mfma_u<1, false>(a, items, s);
+    case 4: return go_small_mfma_u<4, false>(a, items, s);
+    case -4: return go_small_mfma_u<-4, false>(a, items, s);
   }
   return 2;
 }

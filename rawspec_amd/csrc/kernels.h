@@ -38,6 +38,9 @@ struct SmallArgs {
   unsigned nt;
   const float2 *chirp;            // c[n] = exp(-i pi n^2 / nt), n < nt
   const float2 *bsk;              // FFT_N of the wrapped conjugate chirp, over N
+  // k_small_mfma_u with a fused 8-point product: that product's output
+  // [Nds][Q][Nc][8], one dump per 1024-sample spectrum (nullptr: none)
+  float *f8_out;
 };
 
 // Four-step path (N = N1*N2 > 4096), launched per chunk of channels

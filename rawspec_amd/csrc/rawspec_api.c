@@ -470,13 +470,26 @@ static int rs_start(rawspec_context *ctx, const void *d_in, int fft_dir)
 {
   unsigned int i;
   int last[MAX_OUTPUTS];
+  int fa = -1, fb = -1;
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
+  const int fused = rsd_fused_pair(g->dev, &fa, &fb);
   if (rsd_lanes_begin(g->dev)) return 1;
   g->inbuf_count++;
   for (i = 0; i < ctx->No; i++) {
     int first = ((g->inbuf_count - 1) % g->Nis[i]) == 0;
     last[i] = (g->inbuf_count % g->Nis[i]) == 0;
+    if (fused && ((int)i == fa || (int)i == fb)) {
+      /* the pair runs as one, at its first product */
+      if ((int)i != (fa < fb ? fa : fb)) continue;
+      if (rsd_run_fused(g->dev, d_in, fft_dir <= 0,
+                        ((g->inbuf_count - 1) % g->Nis[fb]) == 0)) {
+        fprintf(stderr, "librawspec: processing output products %d and %d failed\n", fa, fb);
+        fflush(stderr);
+        return rs_abandon(g, 1);
+      }
+      continue;
+    }
     if (rsd_run_product(g->dev, (int)i, d_in, fft_dir <= 0, first, last[i])) {
       fprintf(stderr, "librawspec: processing output product %u failed\n", i);
       fflush(stderr);
@@ -497,11 +510,36 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
                           int fft_dir)
 {
   unsigned int i;
+  int fa = -1, fb = -1;
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
+  const int fused = rsd_fused_pair(g->dev, &fa, &fb);
   if (rsd_lanes_begin(g->dev)) return 1;
   for (i = 0; i < ctx->No; i++) {
     unsigned int k = 0;
+    if (fused && ((int)i == fa || (int)i == fb)) {
+      /* the pair runs as one, buffer by buffer, at its first product; each
+       * buffer's dumps follow it in product order */
+      const int lo = fa < fb ? fa : fb, hi = fa < fb ? fb : fa;
+      if ((int)i != lo) continue;
+      for (k = 0; k < nbuf; k++) {
+        const unsigned int n = g->inbuf_count + k;   /* buffers before this one */
+        int dumped = 0, x;
+        if (rsd_run_fused(g->dev, d_in[k], fft_dir <= 0, (n % g->Nis[fb]) == 0)) {
+          fprintf(stderr, "librawspec: processing output products %d and %d failed\n", fa, fb);
+          fflush(stderr);
+          return rs_abandon(g, 0);
+        }
+        for (x = lo; x <= hi; x += hi - lo) {
+          if ((n + 1) % g->Nis[x] != 0 || g->tuning) continue;
+          if (rsd_lane_end(g->dev, x) || rs_dump(ctx, g, (unsigned int)x)) return rs_abandon(g, 0);
+          dumped = 1;
+        }
+        /* the dumps read integration buffers the next buffer rewrites */
+        if (dumped && k + 1 < nbuf && rsd_lane_resume(g->dev, lo)) return rs_abandon(g, 0);
+      }
+      continue;
+    }
     while (k < nbuf) {
       const unsigned int pos = (g->inbuf_count + k) % g->Nis[i];   /* buffers into the integration */
       const unsigned int left = g->Nis[i] - pos;

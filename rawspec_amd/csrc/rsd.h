@@ -69,7 +69,18 @@ int  rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first
  * integration (first: d_in[0] starts it; last: d_in[n-1] completes it). */
 int  rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n, int inverse,
                            int first, int last);
-/* Enqueues the incoherent sum of product i's integration buffer. */
+/* A fused pair under the current engine: 8-point product *a, whose dumps
+ * integrate 128 spectra (one per 1024 samples), computed by the kernel of
+ * 1024-point product *b from the same read of the samples (engine level 1,
+ * int8, two pols, same detection).  Returns 1 and sets a, b when the
+ * context has such a pair and the engine uses it, else 0.  The pair's
+ * products then run together through rsd_run_fused, once per buffer, in
+ * place of rsd_run_product for either. */
+int  rsd_fused_pair(rsd_ctx *d, int *a, int *b);
+/* Enqueues both products of the fused pair over one buffer d_in (NULL: the
+ * context's own input); first_b != 0 starts product b's integration
+ * (product a's integration is always complete within the buffer). */
+int  rsd_run_fused(rsd_ctx *d, const void *d_in, int inverse, int first_b);
 /* Products run on two lanes (streams) forked from the context stream:
  * four-step products on one, small products on the other.  rsd_lanes_begin
  * makes both lanes wait for everything enqueued on the context stream;
@@ -81,6 +92,7 @@ int  rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned 
 int  rsd_lanes_begin(rsd_ctx *d);
 int  rsd_lane_end(rsd_ctx *d, int i);
 int  rsd_lane_resume(rsd_ctx *d, int i);
+/* Enqueues the incoherent sum of product i's integration buffer. */
 int  rsd_run_ics(rsd_ctx *d, int i);
 /* Enqueues the device -> host copy of product i's dump (and ICS dump when
  * h_ics is not NULL). */

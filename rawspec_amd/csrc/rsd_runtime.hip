@@ -235,6 +235,9 @@ struct rsd_ctx {
   float2 *d_tw;
   float *d_w;
   ProdPlan pp[RSD_MAX_PRODUCTS];
+  // a fused pair (-1: none): 8-point product f8_a computed by the kernel of
+  // 1024-point product f8_b when the engine runs that one unsplit
+  int f8_a, f8_b;
   // kernel timing (rsd_profile)
   int mfma;                            // matrix-core engine requested (rsd_set_mfma)
   // allocation-free call in progress (rsd_noalloc_begin .. rsd_noalloc_end):
@@ -562,6 +565,23 @@ static size_t plan_product(rsd_ctx *d, ProdPlan &pp) {
 }
 
 // Sizes and launch plans for a configuration, without touching the device.
+// Two products one kernel computes from a single read of the samples
+// (k_small_mfma_u with its 8-point section): an 8-point product whose dumps
+// integrate 128 spectra, i.e. one dump per 1024 samples and each dump
+// within one buffer, and a 1024-point product of the same detection that
+// the matrix-core engine runs on unsplit operands (int8, two pols).  The
+// 8-point part computes in FP32 as the vector kernel does.  Used at engine
+// level 1, where that product runs unsplit.
+#ifndef RSD_FUSE8
+#define RSD_FUSE8 1
+#endif
+static bool f8_pair(const rsd_ctx *d, const ProdPlan &a, const ProdPlan &b) {
+  return d->cfg.Np == 2 && d->cfg.Nbps == 8 && a.small && !a.bs && a.p.Nt == 8 &&
+         a.p.Na == 128 && a.p.Nis == 1 && a.p.Nds == b.p.Nss && b.small && !b.bs &&
+         b.p.Nt == 1024 && b.m_ok && b.m_unsplit && a.p.npolout == b.p.npolout;
+}
+static bool f8_on(const rsd_ctx *d) { return d->f8_a >= 0 && d->mfma == 1; }
+
 static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
   d->dev = cfg->gpu_index;
   d->cfg = *cfg;
@@ -612,6 +632,14 @@ static int plan_all(const rsd_config *cfg, rsd_ctx *d) {
   d->part_len = part_max;
   d->work_len = work_max;
   d->partb_len = partb_max;
+  d->f8_a = d->f8_b = -1;
+  for (unsigned a = 0; RSD_FUSE8 && a < cfg->No && d->f8_a < 0; ++a)
+    for (unsigned b = 0; b < cfg->No; ++b)
+      if (f8_pair(d, d->pp[a], d->pp[b])) {
+        d->f8_a = (int)a;
+        d->f8_b = (int)b;
+        break;
+      }
   return 0;
 }
 
@@ -775,89 +803,120 @@ static int hold_ready(rsd_ctx *d, ProdPlan &pp);
 static int run_fourstep_held(rsd_ctx *d, int i, const uint8_t *in, float isign, float gam,
                              float scale, int first, int last);
 
+// Small product i over one buffer: its one kernel (transform, detection,
+// integration) and, when a dump group is split over workgroups, the sum of
+// their partial sums.  f8_out: the output of the 8-point product of a
+// fused pair, which the unsplit matrix-core kernel then computes as well.
+static int run_small(rsd_ctx *d, int i, const uint8_t *in, float isign, float gam, float scale,
+                     int first, float *f8_out) {
+  ProdPlan &pp = d->pp[i];
+  const rsd_config &c = d->cfg;
+  const hipStream_t s = prod_stream(d, pp);
+  rsk::SmallArgs a;
+  a.in = in;
+  a.chan_stride = d->chan_stride;
+  a.Nc = c.Nc;
+  a.na_buf = pp.na_buf;
+  a.chunk = pp.chunk;
+  a.G = pp.G;
+  a.Ng = pp.Ng;
+  a.gpw = pp.gpw;
+  a.spg = pp.spg;
+  a.bpc = pp.bpc;
+  a.isign = isign;
+  a.gam = gam;
+  a.scale = scale;
+  a.direct = pp.direct;
+  a.first = first;
+  a.acc = pp.d_acc;
+  a.part = d->d_part;
+  a.tw = d->d_tw;
+  a.nt = pp.p.Nt;
+  a.chirp = pp.d_chirp;
+  a.bsk = pp.d_bsk;
+  a.f8_out = f8_out;
+  // Both engines write the same (host) layout, so a product can change
+  // engine between any two buffers.
+  const bool mf = d->mfma && pp.m_ok;
+  if (f8_out && !(mf && d->mfma == 1 && pp.m_unsplit)) return 1;
+  if (mf) {
+    a.chunk = pp.m_chunk;
+    a.G = pp.m_G;
+    a.gpw = 1;
+    a.direct = pp.m_G == 1;
+    const unsigned long long items = (unsigned long long)c.Nc * pp.Ng * pp.m_G;
+    if (items > 0x7fffffffull) return 1;
+    if (d->mfma == 1 && pp.m_unsplit) {
+      ProfScope ps(d, i, f8_out ? "k_small_mfma_u8" : "k_small_mfma_u", pp.p.Nt, s);
+      if (rsk::launch_small_mfma_u((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
+                                   (unsigned)items, s))
+        return 1;
+    } else {
+      ProfScope ps(d, i, "k_small_mfma", pp.p.Nt, s);
+      if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
+                                 (unsigned)items, s))
+        return 1;
+    }
+  } else {
+    const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
+    if (nb > 0x7fffffffull) return 1;
+    ProfScope ps(d, i, "k_small", pp.p.Nt, s);
+    if (rsk::launch_small((int)pp.bs_m, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
+                          s))
+      return 1;
+  }
+  if (!a.direct) {
+    rsk::ReduceArgs r;
+    r.part = d->d_part;
+    r.acc = pp.d_acc;
+    r.Nc = c.Nc;
+    r.Ng = pp.Ng;
+    r.G = a.G;
+    r.Q = (unsigned)pp.Q;
+    r.N = pp.p.Nt;
+    r.scale = scale;
+    r.first = first;
+    ProfScope ps(d, i, "k_reduce", pp.p.Nt, s);
+    if (rsk::launch_reduce(r, s)) return 1;
+  }
+  return 0;
+}
+
+// Per-call constants: transform direction, the sign of the imaginary cross
+// term (input_conjugated flips it, rawspec_gpu.cu:1120-1123, 1139-1142; the
+// inverse-via-conjugation trick flips it once more) and the SNORM scale.
+struct CallSigns {
+  float isign, gam, scale;
+  CallSigns(const rsd_config &c, int inverse)
+      : isign(inverse ? -1.0f : 1.0f),
+        gam((c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f)),
+        scale(1.0f / ((c.Nbps == 16 ? 32767.0f : 127.0f) * (c.Nbps == 16 ? 32767.0f : 127.0f))) {}
+};
+
 int rsd_run_product(rsd_ctx *d, int i, const void *d_in, int inverse, int first, int last) {
   RSD_CHECK(hipSetDevice(d->dev));
   ProdPlan &pp = d->pp[i];
-  const rsd_config &c = d->cfg;
   const uint8_t *in = d_in ? (const uint8_t *)d_in : d->d_in;
-  const float isign = inverse ? -1.0f : 1.0f;
-  // Im cross term: input_conjugated flips it (rawspec_gpu.cu:1120-1123,
-  // 1139-1142); the inverse-via-conjugation trick flips it once more.
-  const float gam = (c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f);
-  const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
-  const float scale = 1.0f / (full * full);
-  if (pp.small) {
-    const hipStream_t s = prod_stream(d, pp);
-    rsk::SmallArgs a;
-    a.in = in;
-    a.chan_stride = d->chan_stride;
-    a.Nc = c.Nc;
-    a.na_buf = pp.na_buf;
-    a.chunk = pp.chunk;
-    a.G = pp.G;
-    a.Ng = pp.Ng;
-    a.gpw = pp.gpw;
-    a.spg = pp.spg;
-    a.bpc = pp.bpc;
-    a.isign = isign;
-    a.gam = gam;
-    a.scale = scale;
-    a.direct = pp.direct;
-    a.first = first;
-    a.acc = pp.d_acc;
-    a.part = d->d_part;
-    a.tw = d->d_tw;
-    a.nt = pp.p.Nt;
-    a.chirp = pp.d_chirp;
-    a.bsk = pp.d_bsk;
-    // Both engines write the same (host) layout, so a product can change
-    // engine between any two buffers.
-    const bool mf = d->mfma && pp.m_ok;
-    if (mf) {
-      a.chunk = pp.m_chunk;
-      a.G = pp.m_G;
-      a.gpw = 1;
-      a.direct = pp.m_G == 1;
-      const unsigned long long items = (unsigned long long)c.Nc * pp.Ng * pp.m_G;
-      if (items > 0x7fffffffull) return 1;
-      if (d->mfma == 1 && pp.m_unsplit) {
-        ProfScope ps(d, i, "k_small_mfma_u", pp.p.Nt, s);
-        if (rsk::launch_small_mfma_u((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
-                                     (unsigned)items, s))
-          return 1;
-      } else {
-        ProfScope ps(d, i, "k_small_mfma", pp.p.Nt, s);
-        if (rsk::launch_small_mfma((int)pp.p.Nt, (int)c.Nbps, (int)c.Np, pp.p.npolout, a,
-                                   (unsigned)items, s))
-          return 1;
-      }
-    } else {
-      const unsigned long long nb = (unsigned long long)pp.bpc * c.Nc;
-      if (nb > 0x7fffffffull) return 1;
-      ProfScope ps(d, i, "k_small", pp.p.Nt, s);
-      if (rsk::launch_small((int)pp.bs_m, (int)c.Nbps, (int)c.Np, pp.p.npolout, a, (unsigned)nb,
-                            s))
-        return 1;
-    }
-    if (!a.direct) {
-      rsk::ReduceArgs r;
-      r.part = d->d_part;
-      r.acc = pp.d_acc;
-      r.Nc = c.Nc;
-      r.Ng = pp.Ng;
-      r.G = a.G;
-      r.Q = (unsigned)pp.Q;
-      r.N = pp.p.Nt;
-      r.scale = scale;
-      r.first = first;
-      ProfScope ps(d, i, "k_reduce", pp.p.Nt, s);
-      if (rsk::launch_reduce(r, s)) return 1;
-    }
-    return 0;
-  }
+  const CallSigns k(d->cfg, inverse);
+  if (pp.small) return run_small(d, i, in, k.isign, k.gam, k.scale, first, nullptr);
   if (first) pp.holding = hold_ready(d, pp);
-  if (pp.holding) return run_fourstep_held(d, i, in, isign, gam, scale, first, last);
-  return run_fourstep(d, i, &in, 1, isign, gam, scale, first, last);
+  if (pp.holding) return run_fourstep_held(d, i, in, k.isign, k.gam, k.scale, first, last);
+  return run_fourstep(d, i, &in, 1, k.isign, k.gam, k.scale, first, last);
+}
+
+int rsd_fused_pair(rsd_ctx *d, int *a, int *b) {
+  if (!f8_on(d)) return 0;
+  *a = d->f8_a;
+  *b = d->f8_b;
+  return 1;
+}
+
+int rsd_run_fused(rsd_ctx *d, const void *d_in, int inverse, int first_b) {
+  RSD_CHECK(hipSetDevice(d->dev));
+  if (!f8_on(d)) return 1;
+  const uint8_t *in = d_in ? (const uint8_t *)d_in : d->d_in;
+  const CallSigns k(d->cfg, inverse);
+  return run_small(d, d->f8_b, in, k.isign, k.gam, k.scale, first_b, d->pp[d->f8_a].d_acc);
 }
 
 // Four-step product i over n consecutive buffers of one integration: launch
@@ -1091,15 +1150,12 @@ int rsd_run_product_batch(rsd_ctx *d, int i, const void *const *d_in, unsigned n
       if (rsd_run_product(d, i, d_in[j], inverse, first && j == 0, last && j == n - 1)) return 1;
     return 0;
   }
-  const rsd_config &c = d->cfg;
-  const float isign = inverse ? -1.0f : 1.0f;
-  const float gam = (c.conj ? -1.0f : 1.0f) * (inverse ? -1.0f : 1.0f);
-  const float full = c.Nbps == 16 ? 32767.0f : 127.0f;
+  const CallSigns k(d->cfg, inverse);
   // an integration started by a batch is batched throughout; one-buffer
   // calls continuing it run pass A/B per buffer
   if (first) pp.holding = 0;
-  return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, isign, gam,
-                      1.0f / (full * full), first, last);
+  return run_fourstep(d, i, reinterpret_cast<const uint8_t *const *>(d_in), n, k.isign, k.gam,
+                      k.scale, first, last);
 }
 
 // ---- autotuning of the launch plans -------------------------------------
