@@ -108,7 +108,11 @@ int rawspec_amd_set_stream(rawspec_context *ctx, void *stream);
  * the error of an integration falls with its length to a floor near
  * 1.5e-4; hi/lo split operands otherwise), the four-step intermediate in
  * 24-bit floats (about 2e-5 added error at one spectrum per dump), and the
- * vector units for everything else, the four-step passes included.
+ * vector units for everything else, the four-step passes included.  An
+ * 8-point product whose dumps integrate 128 spectra beside an unsplit
+ * 1024-point product of the same detection is computed, in FP32, by that
+ * product's kernel from the same read of the samples; batch calls then run
+ * the two buffer by buffer, each buffer's dumps in product order.
  * enable = 2: every matrix-core kernel with split operands, the four-step
  * first pass among them (error at the FP32 level).  Other
  * values are refused.  A product longer than 4096 points switches engine at
