@@ -9,7 +9,8 @@ import re
 
 import pytest
 
-from tests.spectro_cases import MFMA_SPLIT, MFMA_FLOOR_MULT, PLAIN_TOL, check_results, run_calls
+from tests.spectro_cases import (MFMA_FAST, MFMA_SPLIT, MFMA_FLOOR_MULT, PLAIN_TOL, check_results,
+                                 run_calls, unsplit_products)
 
 pytestmark = pytest.mark.gpu
 
@@ -54,3 +55,16 @@ def test_autotune_matrix_core_engine():
     r, cfg, got, reps = run_calls(["autotune", 2, 1, 1], **kw)
     _check_report(reps[0], kw["Nts"])
     check_results(r, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT)
+
+
+def test_autotune_fused_pair_at_level_1():
+    # at engine level 1 the 8-point and the 1024-point products run as one
+    # kernel: the 1024-point product's candidates retune it, the 8-point
+    # product's leave it as it is; one-buffer and batch calls afterwards
+    kw = dict(Nc=4, Ntpb=65536, Nts=[131072, 8, 1024], Nas=[3, 128, 64], Npolout=[-4, -4, -4],
+              mfma=MFMA_FAST, tone=(1, 20.0, 300.0, 1024.0))
+    r, cfg, got, reps = run_calls(["autotune", 1, 3, 1, 1], **kw)
+    _check_report(reps[0], kw["Nts"])
+    check_results(r, cfg, tol=PLAIN_TOL,
+                  tol_by_product=unsplit_products(kw["Nts"], kw["Nas"], mfma=MFMA_FAST))
+    assert [len(got[i]) for i in range(3)] == [2, 6, 6]
