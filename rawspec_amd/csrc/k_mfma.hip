@@ -390,6 +390,9 @@ __device__ __forceinline__ int f8_slot(int n) { return n + 2 * (n >> 4); }
 // are reduced across lanes and written, FFT-shifted, to a.f8_out.  The
 // transform is always forward: for the inverse, bin k is forward bin -k and
 // the cross term's sign is folded into gam as for the 1024-point product.
+// The 8-point work needs registers, so with F8 the DFT matrix is re-read
+// from LDS per spectrum and the samples are loaded one spectrum ahead (the
+// longer iteration covers the load latency).
 template <int PMODE, bool F8>
 __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a, unsigned items) {
   constexpr int N = 1024;
