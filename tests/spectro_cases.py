@@ -28,6 +28,11 @@ MFMA_SPLIT = 2
 # this fixed ceiling, so an error that grows with the input's FP32 floor
 # still fails.
 FOURSTEP_CEIL = 5e-5
+# k for the FP32 engines.  The reference is one full-length FP32 FFT; a
+# four-step FFT in FP32, even with exact twiddles and scipy's own FP32
+# transforms for both passes, measures 0.7 to 2.6 x its error on the same
+# data (test_tolerance_basis.py), so 3.
+PLAIN_FLOOR_MULT = 3.0
 
 
 def gen_blocks(Nc, Ntpb, Np, Nbps, nblocks, seed=1, sigma=8.0, tone=None, first_blk=0):
@@ -61,7 +66,7 @@ class Fp32Ref:
     detection and integration) for one product.  It measures the error any
     FP32 FFT makes on the same input; for long transforms with a strong tone
     that floor exceeds 1e-5 (rounding spurs of the tone), so four-step
-    products are held to max(PLAIN_TOL, 2 x this floor)."""
+    products are held to max(PLAIN_TOL, PLAIN_FLOOR_MULT x this floor)."""
 
     def __init__(self, cfg, i):
         self.cfg, self.i = cfg, i
@@ -195,12 +200,13 @@ def nonneg_components(mode: int):
     return [0] if mode in (1, -4) else [0, 1]
 
 
-def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0, ceil=FOURSTEP_CEIL,
+def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=PLAIN_FLOOR_MULT, ceil=FOURSTEP_CEIL,
                   tol_by_product=None):
     """Asserts every dump is within tol (four-step products: within
-    min(ceil, max(tol, floor_mult x the FP32 reference's error))); returns
-    max error per product.  tol_by_product {product: bar} overrides tol (and
-    the four-step rule) for the products it names."""
+    min(ceil, max(tol, floor_mult x the FP32 reference's error))), and with
+    ics its incoherent sum within the same bound; returns max error per
+    product.  tol_by_product {product: bar} overrides tol (and the four-step
+    rule) for the products it names."""
     worst = {}
     for (n, i, g, r, gi, ri, f32) in results:
         q = abs(cfg.Npolout[i])
@@ -208,19 +214,20 @@ def check_results(results, cfg, tol=PLAIN_TOL, ics=False, floor_mult=2.0, ceil=F
         e = rel_err(g, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i])
         worst[i] = max(worst.get(i, 0.0), float(e.max()))
         if tol_by_product and i in tol_by_product:
-            assert e.max() <= tol_by_product[i], \
-                f"product {i} buffer {n}: max rel err {e.max():.3e} > {tol_by_product[i]}"
-            continue
-        t = tol
-        if f32 is not None:
-            floor = float(rel_err(f32, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i]).max())
-            t = min(ceil, max(tol, floor_mult * floor))
-            print(f"  product {i}: gpu {e.max():.2e}  fp32-reference floor {floor:.2e}  bound {t:.2e}")
+            t = tol_by_product[i]
+        else:
+            t = tol
+            if f32 is not None:
+                floor = float(rel_err(f32, r, cfg.Nds[i], q, cfg.Nc, cfg.Nts[i], cfg.Npolout[i]).max())
+                t = min(ceil, max(tol, floor_mult * floor))
+                print(f"  product {i}: gpu {e.max():.2e}  fp32-reference floor {floor:.2e}  bound {t:.2e}")
         assert e.max() <= t, f"product {i} buffer {n}: max rel err {e.max():.3e} > {t}"
         if ics:
+            # the incoherent sum is a positive-weighted sum of the antennas'
+            # spectra: held to its product's bound
             ncpa = cfg.Nc // cfg.Nant
             ei = rel_err(gi, ri, cfg.Nds[i], q, ncpa, cfg.Nts[i], cfg.Npolout[i])
-            assert ei.max() <= tol, f"ICS product {i}: max rel err {ei.max():.3e}"
+            assert ei.max() <= t, f"ICS product {i}: max rel err {ei.max():.3e} > {t}"
     return worst
 
 

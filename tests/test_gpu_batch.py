@@ -5,7 +5,8 @@ integrations, on both transform engines."""
 import numpy as np
 import pytest
 
-from tests.spectro_cases import (MFMA_FAST, MFMA_FLOOR_MULT, MFMA_SPLIT, PLAIN_TOL, check_results,
+from tests.spectro_cases import (MFMA_FAST, MFMA_FLOOR_MULT, MFMA_SPLIT, PLAIN_FLOOR_MULT, PLAIN_TOL,
+                                 check_results,
                                  rel_err, run_device_case, unsplit_products)
 
 pytestmark = pytest.mark.gpu
@@ -15,7 +16,7 @@ def _run(**kw):
     print(f"case: {kw}", flush=True)
     res, cfg = run_device_case(**kw)
     worst = check_results(res, cfg, tol=PLAIN_TOL, ics=kw.get("ics", False),
-                          floor_mult=MFMA_FLOOR_MULT if kw.get("mfma") else 2.0,
+                          floor_mult=MFMA_FLOOR_MULT if kw.get("mfma") else PLAIN_FLOOR_MULT,
                           tol_by_product=unsplit_products(kw["Nts"], kw["Nas"], kw.get("Np", 2),
                                                           kw.get("Nbps", 8), kw.get("mfma", 0)))
     print(f"  dumps={len(res)} worst={ {k: f'{v:.2e}' for k, v in worst.items()} }", flush=True)

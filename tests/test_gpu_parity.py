@@ -6,7 +6,8 @@ counts, callback order and channel order must match exactly.
 import numpy as np
 import pytest
 
-from tests.spectro_cases import (MFMA_FAST, MFMA_FLOOR_MULT, MFMA_SPLIT, PLAIN_TOL, check_results,
+from tests.spectro_cases import (MFMA_FAST, MFMA_FLOOR_MULT, MFMA_SPLIT, PLAIN_FLOOR_MULT, PLAIN_TOL,
+                                 check_results,
                                  run_case, unsplit_products)
 
 pytestmark = pytest.mark.gpu
@@ -113,7 +114,7 @@ def test_three_products_full_integration_lengths(mode, mfma):
     res, cfg, ev = run_case(Nc=1, Ntpb=524288, Nts=[1 << 20, 8, 1024], Nas=[nbuf, 128, 3072],
                             Npolout=[mode] * 3, nbuf=nbuf, tone=(0, 20.0, 300.0, 1024.0),
                             mfma=mfma)
-    worst = check_results(res, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT if mfma else 2.0,
+    worst = check_results(res, cfg, tol=PLAIN_TOL, floor_mult=MFMA_FLOOR_MULT if mfma else PLAIN_FLOOR_MULT,
                           tol_by_product=unsplit_products(cfg.Nts[:3], cfg.Nas[:3], mfma=mfma))
     print(f"  dumps={len(res)} worst={ {k: f'{v:.2e}' for k, v in worst.items()} }", flush=True)
     perbin = {}

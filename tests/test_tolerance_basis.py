@@ -1,0 +1,48 @@
+"""The basis of the four-step bar (spectro_cases.PLAIN_FLOOR_MULT): on the
+same seeded data (int8-range noise with a strong tone, two spectra
+integrated, the parity tests' per-bin metric against FP64), a four-step FFT
+computed in FP32 with exact twiddles and scipy's own FP32 transforms for
+both passes is compared with the reference the GPU tests use, one
+full-length FP32 FFT.  Its error reaches more than twice the reference's and
+stays under PLAIN_FLOOR_MULT times it: a multiplier of 2 would fail a
+correct FP32 four-step FFT."""
+import numpy as np
+
+from tests.spectro_cases import PLAIN_FLOOR_MULT
+
+
+def _metric(g, r):
+    return float((np.abs(g - r) / np.maximum(r, r.mean())).max())
+
+
+def _errors(n1, n2, seed, nspec=2):
+    import scipy.fft as sf
+    rng = np.random.default_rng(seed)
+    n = n1 * n2
+    acc64, acc_one, acc_four = np.zeros(n), np.zeros(n), np.zeros(n)
+    t = np.arange(n)
+    for _ in range(nspec):
+        x = np.round(rng.normal(0, 8, n)) + 1j * np.round(rng.normal(0, 8, n))
+        x = x + np.round(20 * np.exp(2j * np.pi * 3.0 * t / n))
+        x = np.clip(x.real, -127, 127) + 1j * np.clip(x.imag, -127, 127)
+        acc64 += np.abs(np.fft.fft(x)) ** 2
+        x32 = x.astype(np.complex64)
+        acc_one += (np.abs(sf.fft(x32)) ** 2).astype(np.float32)
+        # four-step: n2-point FFTs over the stride-n1 columns, twiddle
+        # exp(-2 pi i n1 k2 / n) rounded to FP32, n1-point FFTs over the rows
+        y = sf.fft(x32.reshape(n2, n1), axis=0)
+        k2, c1 = np.arange(n2)[:, None], np.arange(n1)[None, :]
+        y = (y * np.exp(-2j * np.pi * (k2 * c1) / n).astype(np.complex64)).astype(np.complex64)
+        acc_four += (np.abs(sf.fft(y, axis=1).T.reshape(n)) ** 2).astype(np.float32)
+    return _metric(acc_one, acc64), _metric(acc_four, acc64)
+
+
+def test_fp32_four_step_error_against_one_fp32_fft():
+    ratios = []
+    for seed in range(6):
+        one, four = _errors(512, 256, seed)
+        assert one > 0 and four > 0
+        ratios.append(four / one)
+    print("four-step / single FP32 FFT error:", [f"{r:.2f}" for r in ratios])
+    assert max(ratios) > 2.0
+    assert max(ratios) < PLAIN_FLOOR_MULT
