@@ -92,8 +92,10 @@ def test_sixteen_antennas_int16_four_products_ics():
                                                       (1, 2), (1, 3)]
 
 
-@pytest.mark.parametrize("mode,mfma", [(1, 0), (4, 0), (-4, 0), (-4, MFMA_SPLIT), (-4, MFMA_FAST)],
-                         ids=["power", "crosspol", "stokes", "stokes-mfma-split", "stokes-mfma"])
+@pytest.mark.parametrize("mode,mfma", [(1, 0), (4, 0), (-4, 0), (-4, MFMA_SPLIT), (-4, MFMA_FAST),
+                                       (1, MFMA_FAST), (4, MFMA_FAST)],
+                         ids=["power", "crosspol", "stokes", "stokes-mfma-split", "stokes-mfma",
+                              "power-mfma", "crosspol-mfma"])
 def test_three_products_full_integration_lengths(mode, mfma):
     """The 64-channel three-product set at its full integration lengths on
     one of its channels (channels are independent; the tone's): blocks of
@@ -103,7 +105,9 @@ def test_three_products_full_integration_lengths(mode, mfma):
     product over 3072 spectra across 3 buffers; pols 1 (total power), 4
     (XX, YY, Re XY*, Im XY*) and -4 (I, Q, U, V); on the matrix cores, 9
     buffers with Nint {9, 128, 3072}, split operands everywhere, or at the
-    engine's 1e-3 bar (the 1024-point product on unsplit operands).  FP32 sums in blocks (registers, a
+    engine's 1e-3 bar (the 1024-point product on unsplit operands, computed
+    with the 8-point product by one kernel, in each detection mode).  FP32
+    sums in blocks (registers, a
     fixed-order tree across slots, one add per buffer or held group) keep
     the 3072-spectrum integration within the plain bar without compensated
     summation.  Beside the bar's metric, the per-bin |dP|/P of the power
