@@ -26,8 +26,10 @@ MFMA_SPLIT = 2
 # an FP32 FFT on the same input) -- a strong tone puts FP32 rounding spurs
 # above 1e-5 in any single-precision FFT of that length -- and never above
 # this fixed ceiling, so an error that grows with the input's FP32 floor
-# still fails.
-FOURSTEP_CEIL = 5e-5
+# still fails.  The ceiling is about twice the largest error of one
+# full-length FP32 FFT at 2^20 with one spectrum per dump (4.8e-5,
+# test_tolerance_basis.py).
+FOURSTEP_CEIL = 1e-4
 # k for the FP32 engines.  The reference is one full-length FP32 FFT; a
 # four-step FFT in FP32, even with exact twiddles and scipy's own FP32
 # transforms for both passes, measures 0.7 to 2.6 x its error on the same
