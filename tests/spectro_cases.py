@@ -30,11 +30,13 @@ MFMA_SPLIT = 2
 # full-length FP32 FFT at 2^20 with one spectrum per dump (4.8e-5,
 # test_tolerance_basis.py).
 FOURSTEP_CEIL = 1e-4
-# k for the FP32 engines.  The reference is one full-length FP32 FFT; a
-# four-step FFT in FP32, even with exact twiddles and scipy's own FP32
-# transforms for both passes, measures 0.7 to 2.6 x its error on the same
-# data (test_tolerance_basis.py), so 3.
-PLAIN_FLOOR_MULT = 3.0
+# k for the FP32 engines.  The reference is one full-length FP32 FFT; both
+# errors are maxima over all bins, and a four-step FFT in FP32, even with
+# exact twiddles and scipy's own FP32 transforms for both passes, measures
+# a median 1.0 but up to about 4.4 x the reference's error on the same data
+# (test_tolerance_basis.py; the GPU's four-step products: median 0.98, at
+# most 4.1 over 1722 random cases), so 5.
+PLAIN_FLOOR_MULT = 5.0
 
 
 def gen_blocks(Nc, Ntpb, Np, Nbps, nblocks, seed=1, sigma=8.0, tone=None, first_blk=0):
@@ -184,9 +186,10 @@ def run_case(*, Nc, Ntpb, Nts, Nas, Np=2, Nbps=8, Npolout=None, nbuf=1, Nb=0,
 
 # Four-step products on the matrix cores: split-FP16 operands carry 22
 # significant bits against FP32's 24, so with a strong tone their rounding
-# spurs reach a few times those of an FP32 FFT (still far below the 1e-3
-# bar of the matrix-core engine).
-MFMA_FLOOR_MULT = 4.0
+# spurs reach about 1.3 x those of the FP32 engine (2.0 against 1.5 x the
+# FP32 reference at 2^20; still far below the 1e-3 bar of the matrix-core
+# engine): the FP32 engine's factor, one more.
+MFMA_FLOOR_MULT = PLAIN_FLOOR_MULT + 1.0
 
 
 def bin_rel_err(gpu: np.ndarray, ref: np.ndarray) -> float:

@@ -1,14 +1,13 @@
 """The basis of the four-step bar (spectro_cases.PLAIN_FLOOR_MULT and
-FOURSTEP_CEIL): on the
-same seeded data (int8-range noise with a strong tone, two spectra
-integrated, the parity tests' per-bin metric against FP64), a four-step FFT
-computed in FP32 with exact twiddles and scipy's own FP32 transforms for
-both passes is compared with the reference the GPU tests use, one
-full-length FP32 FFT.  Its error reaches more than twice the reference's and
-stays under PLAIN_FLOOR_MULT times it: a multiplier of 2 would fail a
-correct FP32 four-step FFT.  At 2^20 with one spectrum per dump the
-reference's own error reaches close to 5e-5, so the fixed ceiling sits at
-twice that."""
+FOURSTEP_CEIL).  Both the GPU's four-step error and the reference it is
+judged against (one full-length FP32 FFT, scipy on complex64) are maxima of
+the parity tests' per-bin metric over all bins, so their ratio is a noisy
+statistic.  On seeded data (int8-range noise with a strong tone, one or two
+spectra integrated) a four-step FFT computed in FP32 with exact twiddles and
+scipy's own FP32 transforms for both passes has a median ratio of 1 but a
+tail well past 2: the factor must cover that tail.  At 2^20 with one
+spectrum per dump the reference's own error reaches close to 5e-5, so the
+fixed ceiling sits at twice that."""
 import numpy as np
 
 from tests.spectro_cases import FOURSTEP_CEIL, PLAIN_FLOOR_MULT
@@ -42,13 +41,16 @@ def _errors(n1, n2, seed, nspec=2):
 
 def test_fp32_four_step_error_against_one_fp32_fft():
     ratios = []
-    for seed in range(6):
-        one, four = _errors(512, 256, seed)
+    for seed in range(300):
+        one, four = _errors(512, 256, seed, nspec=1 + seed % 2)
         assert one > 0 and four > 0
         ratios.append(four / one)
-    print("four-step / single FP32 FFT error:", [f"{r:.2f}" for r in ratios])
-    assert max(ratios) > 2.0
-    assert max(ratios) < PLAIN_FLOOR_MULT
+    ratios.sort()
+    median, p90, top = ratios[len(ratios) // 2], ratios[int(0.9 * len(ratios))], ratios[-1]
+    print(f"four-step / single FP32 FFT error: median {median:.2f}, p90 {p90:.2f}, max {top:.2f}")
+    assert 0.8 < median < 1.25           # the same accuracy in the bulk
+    assert top > 4.0                     # a factor of 2, 3 or 4 fails correct cases
+    assert top < PLAIN_FLOOR_MULT
 
 
 def test_single_spectrum_fp32_error_at_2_20_against_the_ceiling():
