@@ -1,5 +1,7 @@
-"""Seeded random configurations through the reference API (host blocks,
-rawspec_start_processing) against the FP64 oracle: transform lengths that
+"""Seeded random configurations against the FP64 oracle, through the
+reference API (host blocks, rawspec_start_processing) and through the
+batched device entry point (random batch sizes, two-lane execution forced on
+or off, or left to the library): transform lengths that
 are powers of two and other lengths (each a divisor of the longest), one or
 two pols, int8 or int16, every polarisation mode, forward and inverse,
 conjugated input, several channels, the incoherent sum over antennas,
@@ -11,7 +13,7 @@ import pytest
 
 import oracle as O
 from tests.spectro_cases import (MFMA_FLOOR_MULT, PLAIN_FLOOR_MULT, PLAIN_TOL, check_results, run_case,
-                                 unsplit_products)
+                                 run_device_case, unsplit_products)
 
 pytestmark = pytest.mark.gpu
 
@@ -58,6 +60,26 @@ def test_random_configuration(seed):
     kw = _config(seed)
     print(f"case: seed {seed} {kw}", flush=True)
     res, cfg, _ = run_case(**kw)
+    level = kw["mfma"]
+    check_results(res, cfg, tol=PLAIN_TOL, ics=kw.get("ics", False),
+                  floor_mult=MFMA_FLOOR_MULT if level == 2 else PLAIN_FLOOR_MULT,
+                  tol_by_product=unsplit_products(kw["Nts"], kw["Nas"], kw["Np"], kw["Nbps"],
+                                                  mfma=level))
+    assert res, "no dump"
+
+
+@pytest.mark.parametrize("seed", range(100))
+def test_random_configuration_batched(seed):
+    kw = _config(seed)
+    rng = random.Random(seed)
+    nbuf = max(kw.pop("nbuf"), rng.choice([2, 4, 6, 8]))
+    batches, left = [], nbuf
+    while left:
+        batches.append(rng.randint(1, left))
+        left -= batches[-1]
+    lanes = rng.choice([None, True, False])
+    print(f"case: seed {seed} batches {batches} lanes {lanes} {kw}", flush=True)
+    res, cfg = run_device_case(nbuf=nbuf, batches=batches, lanes=lanes, **kw)
     level = kw["mfma"]
     check_results(res, cfg, tol=PLAIN_TOL, ics=kw.get("ics", False),
                   floor_mult=MFMA_FLOOR_MULT if level == 2 else PLAIN_FLOOR_MULT,
