@@ -269,6 +269,12 @@ def other_config(torch, R, W, nring=8):
     dt8, _ = timed(torch, sp, ring, sb, warmup=8, batch=8)
     out["served_GBps"] = round(sb * buf_bytes / dt8 / 1e9, 2)
     out["served_ms_per_step"] = round(dt8 / sb * 1e3, 4)
+    # the same on the matrix-core engine at its 1e-3 bar (integrations
+    # restarted so the leg runs only its own engine), then back to FP32
+    assert sp.set_mfma(1) == 0 and sp.reset_integration() == 0
+    dtm, _ = timed(torch, sp, ring, sb, warmup=8, batch=8)
+    out["served_mfma_GBps"] = round(sb * buf_bytes / dtm / 1e9, 2)
+    assert sp.set_mfma(0) == 0 and sp.reset_integration() == 0
     # the launch-plan autotuner for batches of 8, then the batches again
     t1 = time.perf_counter()
     rep = sp.autotune(ring[0].data_ptr(), 8)

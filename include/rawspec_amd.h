@@ -102,11 +102,11 @@ int rawspec_amd_set_stream(rawspec_context *ctx, void *stream);
 /* Transform engine of the context.  enable = 0: FP32 on the vector units
  * (the default; 1e-5 relative error bar).  enable = 1: the matrix-core engine
  * at its 1e-3 relative error bar, using each kernel where it is faster: FP16
- * matrix products with FP32 accumulation for 1024-point products (unsplit
- * operands when an int8 product integrates at least 32 spectra per dump:
- * about 2^-12 rounding per transform, random from spectrum to spectrum, so
- * the error of an integration falls with its length to a floor near
- * 1.5e-4; hi/lo split operands otherwise), the four-step intermediate in
+ * matrix products with FP32 accumulation for 1024-point int8 products that
+ * integrate at least 32 spectra per dump (unsplit operands: about 2^-12
+ * rounding per transform, random from spectrum to spectrum, so the error of
+ * an integration falls with its length to a floor near 1.5e-4; shorter
+ * integrations stay on the vector units), the four-step intermediate in
  * 24-bit floats (about 2e-5 added error at one spectrum per dump), and the
  * vector units for everything else, the four-step passes included.  An
  * 8-point product whose dumps integrate 128 spectra beside an unsplit

@@ -338,7 +338,8 @@ class Spectrometer:
 
     def set_mfma(self, level) -> int:
         """0 / False: vector units; 1 / True: matrix cores at the 1e-3 bar
-        (unsplit operands for long int8 integrations); 2: matrix cores with
+        (unsplit operands for long int8 integrations, the vector units
+        where they are faster); 2: matrix cores with
         split operands everywhere."""
         return lib().rawspec_amd_set_mfma(C.byref(self.ctx), int(level))
 

@@ -128,8 +128,8 @@ int rsd_profile_report(rsd_ctx *d, char *buf, size_t len);
 
 /* Transform engine: 0 = FP32 vector units (default); 1 = the matrix-core
  * engine at its 1e-3 bar (unsplit FP16 operands for int8 1024-point products
- * integrating at least 32 spectra, split elsewhere, a 24-bit four-step
- * intermediate, vector kernels where faster, both four-step passes included;
+ * integrating at least 32 spectra, a 24-bit four-step intermediate, vector
+ * kernels everywhere else, both four-step passes included;
  * see rsd_fused_pair for the 8-point product it computes with a 1024-point one);
  * 2 = matrix cores with split FP16 operands everywhere (FP32-level error).
  * Other levels are refused (1).  A four-step product switches at the start

@@ -836,9 +836,13 @@ static int run_small(rsd_ctx *d, int i, const uint8_t *in, float isign, float ga
   a.bsk = pp.d_bsk;
   a.f8_out = f8_out;
   // Both engines write the same (host) layout, so a product can change
-  // engine between any two buffers.
-  const bool mf = d->mfma && pp.m_ok;
-  if (f8_out && !(mf && d->mfma == 1 && pp.m_unsplit)) return 1;
+  // engine between any two buffers.  Level 2 runs every matrix-core kernel;
+  // level 1 only the unsplit one: the split kernel is never faster than the
+  // vector kernel (0.204 against 0.192 ms per 256 MiB buffer of 64
+  // channels; 2.6 times slower on 4 MiB buffers of 8 channels), which
+  // meets the tighter FP32 bar as well.
+  const bool mf = pp.m_ok && (d->mfma == 2 || (d->mfma == 1 && pp.m_unsplit));
+  if (f8_out && !(mf && d->mfma == 1)) return 1;
   if (mf) {
     a.chunk = pp.m_chunk;
     a.G = pp.m_G;

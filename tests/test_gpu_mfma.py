@@ -8,8 +8,9 @@ times the error of an FP32 reference FFT (metric in spectro_cases.py).
 
 Level 1 (MFMA_FAST, the engine at its own 1e-3 bar): 1024-point int8
 products integrating >= 32 spectra run with unsplit FP16 operands and are
-held to 1e-3 (their measured errors are printed); everything else keeps split
-operands and the tight bounds.
+held to 1e-3 (their measured errors are printed); shorter 1024-point
+integrations run the vector kernel, and everything else keeps the tight
+bounds.
 
 The matrix-core kernels cover the 1024-point small products and the
 four-step lengths whose pass B is 1024 points (2^19, 2^20), with two pols;
@@ -130,8 +131,11 @@ def test_unsplit_kernel_selection():
     kw = dict(Nc=2, Ntpb=65536, Nts=[1024], Npolout=[-4])
     print("case: kernel selection by level and integration length", flush=True)
     assert "k_small_mfma_u" in _kernels(MFMA_FAST, Nas=[64], **kw)
-    assert "k_small_mfma_u" not in _kernels(MFMA_FAST, Nas=[16], **kw)   # too short: split
-    assert "k_small_mfma_u" not in _kernels(MFMA_SPLIT, Nas=[64], **kw)
+    # too short for unsplit operands: the vector kernel (faster than split)
+    short = _kernels(MFMA_FAST, Nas=[16], **kw)
+    assert "k_small" in short and not any(k.startswith("k_small_mfma") for k in short), short
+    split = _kernels(MFMA_SPLIT, Nas=[64], **kw)
+    assert "k_small_mfma" in split and "k_small_mfma_u" not in split, split
     assert "k_small" in _kernels(0, Nas=[64], **kw)
 
 
