@@ -803,6 +803,15 @@ static int hold_ready(rsd_ctx *d, ProdPlan &pp);
 static int run_fourstep_held(rsd_ctx *d, int i, const uint8_t *in, float isign, float gam,
                              float scale, int first, int last);
 
+// Whether small product pp runs on the matrix cores under the current
+// engine.  Level 2 runs every matrix-core kernel; level 1 only the unsplit
+// one: the split kernel is never faster than the vector kernel (0.204
+// against 0.192 ms per 256 MiB buffer of 64 channels; 2.6 times slower on
+// 4 MiB buffers of 8 channels), which meets the tighter FP32 bar as well.
+static bool small_on_mfma(const rsd_ctx *d, const ProdPlan &pp) {
+  return pp.m_ok && (d->mfma == 2 || (d->mfma == 1 && pp.m_unsplit));
+}
+
 // Small product i over one buffer: its one kernel (transform, detection,
 // integration) and, when a dump group is split over workgroups, the sum of
 // their partial sums.  f8_out: the output of the 8-point product of a
@@ -836,12 +845,8 @@ static int run_small(rsd_ctx *d, int i, const uint8_t *in, float isign, float ga
   a.bsk = pp.d_bsk;
   a.f8_out = f8_out;
   // Both engines write the same (host) layout, so a product can change
-  // engine between any two buffers.  Level 2 runs every matrix-core kernel;
-  // level 1 only the unsplit one: the split kernel is never faster than the
-  // vector kernel (0.204 against 0.192 ms per 256 MiB buffer of 64
-  // channels; 2.6 times slower on 4 MiB buffers of 8 channels), which
-  // meets the tighter FP32 bar as well.
-  const bool mf = pp.m_ok && (d->mfma == 2 || (d->mfma == 1 && pp.m_unsplit));
+  // engine between any two buffers.
+  const bool mf = small_on_mfma(d, pp);
   if (f8_out && !(mf && d->mfma == 1)) return 1;
   if (mf) {
     a.chunk = pp.m_chunk;
@@ -1200,7 +1205,7 @@ static void set_tunables(ProdPlan &pp, const Tunables &t) {
 static std::vector<Tunables> tune_cands(const rsd_ctx *d, const ProdPlan &pp) {
   const Tunables dflt = {2048u, 16u, 2048u, kFsWorkBytes};
   std::vector<Tunables> c = {dflt};
-  if (pp.small && d->mfma && pp.m_ok) {
+  if (pp.small && small_on_mfma(d, pp)) {
     for (unsigned tg : {512u, 1024u, 4096u, 8192u}) c.push_back({dflt.target, dflt.maxspg, tg, dflt.budget});
   } else if (pp.small) {
     for (unsigned tg : {512u, 1024u, 2048u, 4096u, 8192u})
