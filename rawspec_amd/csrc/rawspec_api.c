@@ -524,13 +524,14 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
       if ((int)i != lo) continue;
       for (k = 0; k < nbuf; k++) {
         const unsigned int n = g->inbuf_count + k;   /* buffers before this one */
-        int dumped = 0, x;
+        int dumped = 0, j;
         if (rsd_run_fused(g->dev, d_in[k], fft_dir <= 0, (n % g->Nis[fb]) == 0)) {
           fprintf(stderr, "librawspec: processing output products %d and %d failed\n", fa, fb);
           fflush(stderr);
           return rs_abandon(g, 0);
         }
-        for (x = lo; x <= hi; x += hi - lo) {
+        for (j = 0; j < 2; j++) {
+          const int x = j ? hi : lo;
           if ((n + 1) % g->Nis[x] != 0 || g->tuning) continue;
           if (rsd_lane_end(g->dev, x) || rs_dump(ctx, g, (unsigned int)x)) return rs_abandon(g, 0);
           dumped = 1;
