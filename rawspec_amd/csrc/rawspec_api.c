@@ -451,6 +451,17 @@ static int rs_dump(rawspec_context *ctx, rs_gpu_ctx *g, unsigned int i)
   return 0;
 }
 
+/* Whether a dump queues work on the context stream: an incoherent sum, the
+ * copies into the host buffers or a callback.  Device-resident output with
+ * no callback and no incoherent sum queues nothing, so a batch call need not
+ * join the product's lane into the context stream at each such dump (a join
+ * followed by the lane's resume would also hold the lane behind everything
+ * the context stream waited for, the other lane's work included). */
+static int rs_dump_queues(const rawspec_context *ctx, const rs_gpu_ctx *g)
+{
+  return ctx->incoherently_sum || !g->device_output || ctx->dump_callback != NULL;
+}
+
 /* A start call that fails after forking the lanes joins them back into the
  * context stream (so rawspec_wait_for_completion() also waits for the
  * kernels already queued there, which read the input buffer) and takes back
@@ -514,6 +525,7 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
   rs_gpu_ctx *g = rs_get(ctx);
   if (!g) return 1;
   const int fused = rsd_fused_pair(g->dev, &fa, &fb);
+  const int dumps = !g->tuning && rs_dump_queues(ctx, g);
   if (rsd_lanes_begin(g->dev)) return 1;
   for (i = 0; i < ctx->No; i++) {
     unsigned int k = 0;
@@ -532,7 +544,7 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
         }
         for (j = 0; j < 2; j++) {
           const int x = j ? hi : lo;
-          if ((n + 1) % g->Nis[x] != 0 || g->tuning) continue;
+          if ((n + 1) % g->Nis[x] != 0 || !dumps) continue;
           if (rsd_lane_end(g->dev, x) || rs_dump(ctx, g, (unsigned int)x)) return rs_abandon(g, 0);
           dumped = 1;
         }
@@ -552,7 +564,7 @@ static int rs_start_batch(rawspec_context *ctx, const void *const *d_in, unsigne
         return rs_abandon(g, 0);
       }
       /* the dump reads the integration buffer the lane's next piece rewrites */
-      if (last && !g->tuning &&
+      if (last && dumps &&
           (rsd_lane_end(g->dev, (int)i) || rs_dump(ctx, g, i) ||
            (k + len < nbuf && rsd_lane_resume(g->dev, (int)i))))
         return rs_abandon(g, 0);
