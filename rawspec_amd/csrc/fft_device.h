@@ -456,13 +456,15 @@ __device__ __forceinline__ float2 load_y(const float2 *p) {
 }
 
 // 24-bit intermediate (engine level 1, two pols): each float of Y is rounded
-// to nearest even at mantissa bit 8 and its upper three bytes kept, so a
-// two-pol complex value takes 12 bytes instead of 16 (every value keeps its
-// own exponent; 16 significant bits).  Packing and unpacking are byte
-// permutes.
+// to nearest at mantissa bit 8 (ties away from zero) and its upper three
+// bytes kept, so a two-pol complex value takes 12 bytes instead of 16 (every
+// value keeps its own exponent; 16 significant bits).  Adding half a unit of
+// the kept bits to the sign-magnitude pattern rounds the magnitude; a carry
+// out of the mantissa steps the exponent, as rounding up to the next binade
+// needs.  One add per value (ties to even took a bit extract and a
+// three-operand add); packing and unpacking are byte permutes.
 __device__ __forceinline__ unsigned int round24(float x) {
-  const unsigned int b = __float_as_uint(x);
-  return b + 0x7fu + ((b >> 8) & 1u);   // the upper three bytes hold the rounded value
+  return __float_as_uint(x) + 0x80u;   // the upper three bytes hold the rounded value
 }
 // One 12-byte access per value (global_{load,store}_dwordx3 at 4-byte
 // alignment): lanes holding consecutive values then cover a contiguous run
