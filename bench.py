@@ -12,10 +12,11 @@ One step = one device buffer, with the input already resident in HBM: a
 ring of 64 buffers (16 GiB, the whole 128-block run) is generated on the
 device before timing and cycled.  Dumps stay in device memory
 (rawspec_amd_set_device_output), so PCIe is outside the timed region; the
-PCIe-inclusive figure is reported separately (--pcie).  Three paths are
+PCIe-inclusive figure is reported separately (--pcie).  Four paths are
 timed: "plain", one buffer per rawspec_amd_start_processing_device() call
 (the granularity of the reference's rawspec_start_processing()) on the FP32
-vector units; "served", batches of 8 consecutive buffers per
+vector units; "plain_mfma", the same calls on the matrix-core engine;
+"served", batches of 8 consecutive buffers per
 rawspec_amd_start_processing_device_batch() call; and "served_mfma", the
 same on the matrix-core engine.  `value` is the faster served path (both
 keep the plain path's error bar).  paths.served_autotuned is the served
@@ -403,7 +404,7 @@ def main(argv=None):
     ap.add_argument("--ingest-blocks", type=int, default=32,
                     help="blocks of the GUPPI RAW file the rawspec program ingests (0: skip)")
     ap.add_argument("--pcie", action="store_true", help="also time the host-buffer API path")
-    ap.add_argument("--legs", default="plain,served,served_mfma,served_autotuned",
+    ap.add_argument("--legs", default="plain,plain_mfma,served,served_mfma,served_autotuned",
                     help="paths to time (comma-separated subset of the default)")
     ap.add_argument("--lanes", choices=["auto", "on", "off"], default="auto",
                     help="two-lane execution of the headline context (auto: the library's "
@@ -482,7 +483,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         del src, dst
     want = [x for x in a.legs.split(",") if x]
-    spec = {"plain": (1, False), "served": (a.batch, False), "served_mfma": (a.batch, True)}
+    spec = {"plain": (1, False), "plain_mfma": (1, True), "served": (a.batch, False),
+            "served_mfma": (a.batch, True)}
     legs = {k: measure(*spec[k]) for k in spec if k in want}
     if "plain" not in legs:
         legs["plain"] = legs[want[0]]
