@@ -406,6 +406,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   uint2 *t8 = reinterpret_cast<uint2 *>(lds_raw + sizeof(Dft1024U)) + wv * kF8Tile;
   const int col = lane & 31, h = lane >> 5;
+  uint2 *const t8w = t8 + f8_slot(col + 128 * h);   // this lane's first tile store
   const unsigned item = blockIdx.x * kMfmaWaves + wv;
   const bool valid = item < items;
   const unsigned it = valid ? item : 0u;
@@ -466,7 +467,9 @@ __global__ __launch_bounds__(64 * kMfmaWaves, 2) void k_small_mfma_u(SmallArgs a
       ah[0][e >> 2][2 * (e & 3) + 1] = p0[1];
       ah[1][e >> 2][2 * (e & 3)] = p1[0];
       ah[1][e >> 2][2 * (e & 3) + 1] = p1[1];
-      if constexpr (F8) t8[f8_slot(col + 32 * (8 * (e >> 2) + 4 * h + (e & 3)))] = make_uint2(u0, u1);
+      // f8_slot(col + 128 h + 32 K) = f8_slot(col + 128 h) + 36 K (32 K is a
+      // multiple of 16), so every store is one base plus a constant
+      if constexpr (F8) t8w[36 * (8 * (e >> 2) + (e & 3))] = make_uint2(u0, u1);
     }
     if constexpr (!F8) {
 #pragma unroll
