@@ -107,10 +107,13 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
     const unsigned cb = cs / a.Nss;
     const unsigned b = cb % a.nbat;    // buffer within the batch
     const unsigned cl = cb / a.nbat;   // channel within the chunk
+    // a base common to the workgroup and 32-bit element offsets per lane
+    // (below N <= 2^24), so each load adds one constant to one offset
     const W *src = reinterpret_cast<const W *>(a.in[b] + (unsigned long long)(a.c0 + cl) * a.chan_stride) +
-                   (unsigned long long)sp * N + tile * TA + sg;
+                   (unsigned long long)sp * N + tile * TA;
+    const unsigned o = (unsigned)sg + (unsigned)j * N1;
 #pragma unroll
-    for (int m = 0; m < E; ++m) raw[m] = src[(unsigned long long)(j + m * T) * N1];
+    for (int m = 0; m < E; ++m) raw[m] = src[o + (unsigned)(m * T) * N1];
   };
   unsigned l = xcd_remap(blockIdx.x, gridDim.x);
   W raw[E];
@@ -155,7 +158,7 @@ __global__ __launch_bounds__(wg_team(N2)) void k_fs_a(FsArgs a) {
       if constexpr (NP == 2) {
         const float2 y0 = cmul(v[0][m], w), y1 = cmul(v[1][m], w);
         const float4 yv = make_float4(y0.x, y0.y, y1.x, y1.y);
-        if constexpr (Y24) store_y24(reinterpret_cast<unsigned int *>(dst) + 3ull * off, yv);
+        if constexpr (Y24) store_y24(reinterpret_cast<unsigned int *>(dst) + 3u * off, yv);   // 3 off < 2^26
         else store_y(reinterpret_cast<float4 *>(dst) + off, yv);
       } else {
         store_y(dst + off, cmul(v[0][m], w));
